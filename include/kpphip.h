@@ -1,0 +1,146 @@
+/* kpphip.h — C ABI of the MI355X batched Rosenbrock solver for KPP mechanisms.
+ *
+ * Drop-in boundary for KPP's C integration path.  KPP's reference surface
+ * integrates ONE cell per call over global state:
+ *   void INTEGRATE(double TIN, double TOUT)                int/rosenbrock.c:94-133
+ *   int  Rosenbrock(double Y[], double Tstart, double Tend,
+ *                   double AbsTol[], double RelTol[], ode_Fun, ode_Jac,
+ *                   double RPAR[], int IPAR[])             int/rosenbrock.c:137-446
+ * and the generated model ABI Fun/Jac_SP/KppDecomp/KppSolve/Update_RCONST
+ * (prototypes int/rosenbrock.c:54-91; generated by src/gen.c:642-760,
+ * 1185-1319, 1960-2033, 2343-2473; KppDecomp util/sutil.c:2-23).
+ * The functions below do the same for many independent cells at once on one
+ * GPU.  IPAR/RPAR keep the index meanings of int/rosenbrock.c:201-254:
+ *   IPAR[0] autonomous, IPAR[1] scalar tolerances (0 = vector), IPAR[2] max
+ *   steps, IPAR[3] method (1 Ros2, 2 Ros3, 3 Ros4, 4 Rodas3, 5 Rodas4),
+ *   IPAR[10..17] Nfun Njac Nstp Nacc Nrej Ndec Nsol Nsng on entry;
+ *   RPAR[0..6] Hmin Hmax Hstart FacMin FacMax FacRej FacSafe.
+ * Per-cell results use Rosenbrock's codes (1 success, -1..-8 failures);
+ * per-cell statistics come back in the IPAR[10..17] order, Texit/Hexit in
+ * the RPAR[10..11] order.  Parameter errors (-1..-5) are returned by the
+ * call itself, as Rosenbrock returns them before integrating.  API misuse
+ * returns KPP_E_* (<= -100).  Nothing is printed.
+ *
+ * Arrays are structure-of-arrays: element s of cell c is X[s*ld + c].
+ * Concentrations C hold NSPEC rows: NVAR variable species, then NFIX fixed
+ * species, in the mechanism's KPP order (kpp_mech_species()).
+ */
+#ifndef KPPHIP_H
+#define KPPHIP_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KPP_E_MECH (-101)      /* unknown mechanism name */
+#define KPP_E_ARG (-102)       /* bad argument (null pointer, size) */
+#define KPP_E_HIP (-103)       /* HIP runtime error */
+#define KPP_E_WORKSPACE (-104) /* workspace too small */
+#define KPP_E_NODEV (-105)     /* no GPU device */
+
+#define KPP_SUN_FIXED 0 /* SUN given per cell and held fixed (Update_SUN bypassed) */
+#define KPP_SUN_TOD 1   /* SUN from Update_SUN(TIME) (util/UpdateSun.c:1-20) */
+
+#define KPP_VARIANT_PLAIN 0      /* one lane per cell, cells in launch order */
+#define KPP_VARIANT_PERSISTENT 1 /* persistent work-stealing wavefronts */
+
+/* ---- mechanisms compiled into the library ---- */
+int kpp_mech_count(void);
+const char* kpp_mech_name(int i);
+/* sizes: out[0..7] = NVAR NFIX NSPEC NREACT NONZERO LU_NONZERO
+ *        RATES_USE_TIME N_SUN_RATES (ROOT_Parameters.h, src/gen.c:2627-2720) */
+int kpp_mech_info(const char* mech, int out[8]);
+const char* kpp_mech_species(const char* mech, int k);
+/* LU_ICOL[LU_NONZERO], LU_CROW[NVAR+1], LU_DIAG[NVAR+1] (ROOT_JacobianSP,
+ * src/gen.c:530-588), 0-based */
+int kpp_mech_lu_pattern(const char* mech, int* icol, int* crow, int* diag);
+
+/* Rosenbrock's parameter checks (int/rosenbrock.c:286-382) on host data:
+ * returns 0 if valid, else -1..-5 exactly as Rosenbrock would. */
+int kpp_check_params(const int IPAR[20], const double RPAR[20], const double* atol,
+                     const double* rtol, int nvar);
+
+/* ---- device-resident batch (inputs already in GPU memory) ---- */
+typedef struct {
+  int ncells;            /* number of cells */
+  int ld;                /* leading dimension of every SoA array (>= ncells) */
+  double* C;             /* device [NSPEC][ld] concentrations, in/out */
+  const double* temp;    /* device [ncells] TEMP (K) */
+  const double* cfactor; /* device [ncells] CFACTOR (M/1e6 for saprc99 rate laws) */
+  const double* sun;     /* device [ncells] SUN in fixed mode, else NULL */
+  int sun_mode;          /* KPP_SUN_FIXED or KPP_SUN_TOD */
+  const double* tstart;  /* device [ncells] or NULL (then tstart0) */
+  const double* tend;    /* device [ncells] or NULL (then tend0) */
+  double tstart0, tend0;
+  const double* hstart;  /* device [ncells] starting step per cell (>=0) or NULL (RPAR[2]) */
+  const double* rconst;  /* device [NREACT][ld] rate constants supplied by the caller or NULL */
+  int* ierr;             /* device [ncells] Rosenbrock return code per cell */
+  int* istat;            /* device [8][ld] IPAR[10..17] per cell, or NULL */
+  double* rstat;         /* device [2][ld] Texit, Hexit per cell, or NULL */
+  const int* order;      /* device [ncells] processing order (a permutation) or NULL */
+} kpp_batch;
+
+typedef struct {
+  int variant;    /* KPP_VARIANT_PLAIN / KPP_VARIANT_PERSISTENT */
+  int nblocks;    /* persistent: wavefronts in the grid (0 = library default) */
+  int refill_min; /* persistent: refill when this many lanes are idle (0 = default) */
+} kpp_launch;
+
+/* Device bytes of workspace needed for a launch. */
+size_t kpp_workspace_bytes(const char* mech, int method, const kpp_launch* l, int ncells);
+
+/* Batched Rosenbrock (int/rosenbrock.c:137-627) on device-resident data.
+ * atol/rtol are HOST arrays of NVAR entries (IPAR[1]==0) or 1 entry.
+ * workspace: device memory of kpp_workspace_bytes() bytes; stream: a
+ * hipStream_t (NULL = default stream).  Asynchronous: results are ready
+ * when the stream is.  Returns 0, a parameter error -1..-5, or KPP_E_*. */
+int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
+                       const double RPAR[20], const double* atol, const double* rtol,
+                       const kpp_launch* l, void* workspace, size_t workspace_bytes,
+                       void* stream);
+
+/* ---- batched generated-model functions (device SoA arrays, ld >= n) ---- */
+/* Fun(V,F,RCT,Vdot)                 src/gen.c:642-760 */
+int kpp_fun_dev(const char* mech, int n, int ld, const double* V, const double* F,
+                const double* RCT, double* Vdot, void* stream);
+/* Jac_SP(V,F,RCT,JVS)               src/gen.c:1185-1319 */
+int kpp_jac_sp_dev(const char* mech, int n, int ld, const double* V, const double* F,
+                   const double* RCT, double* JVS, void* stream);
+/* KppDecomp(JVS) -> ising per cell  util/sutil.c:2-23 */
+int kpp_decomp_dev(const char* mech, int n, int ld, double* JVS, int* ising, void* stream);
+/* KppSolve(JVS,X)                   src/gen.c:1960-2033 */
+int kpp_solve_dev(const char* mech, int n, int ld, const double* JVS, double* X, void* stream);
+/* Update_RCONST at per-cell TEMP/CFACTOR/SUN/TIME (src/gen.c:2343-2473;
+ * rate laws util/UserRateLaws.c:9-78).  NUMBER-type rates are constants
+ * inside Fun/Jac_SP and are written as 0. */
+int kpp_rates_dev(const char* mech, int n, int ld, const double* temp, const double* cfactor,
+                  const double* sun, const double* time, double* RCONST, void* stream);
+
+/* ---- host-buffer convenience API (allocates device memory itself) ---- */
+typedef struct kpp_ctx kpp_ctx;
+/* method: IPAR[3] code; flags: KPP_SUN_TOD to drive SUN by time of day */
+int kpp_ctx_create(kpp_ctx** ctx, const char* mech, int ncells, int method, int flags);
+void kpp_ctx_destroy(kpp_ctx* ctx);
+/* host SoA C[NSPEC][ncells]; per-cell temp, cfactor, sun (NULL in TOD mode) */
+int kpp_set_state(kpp_ctx* ctx, const double* C, const double* temp, const double* cfactor,
+                  const double* sun);
+/* Rosenbrock over all cells, host arrays; tstart/tend per cell; hstart per
+ * cell or NULL; IPAR[3] must equal the context's method. Synchronous. */
+int kpp_rosenbrock_batch(kpp_ctx* ctx, const double* tstart, const double* tend,
+                         const double* atol, const double* rtol, const int IPAR[20],
+                         const double RPAR[20], const double* hstart);
+/* results to host: C[NSPEC][ncells], ierr[ncells], istat[8][ncells], rstat[2][ncells]
+ * (any may be NULL) */
+int kpp_get_state(kpp_ctx* ctx, double* C, int* ierr, int* istat, double* rstat);
+/* INTEGRATE(TIN,TOUT) for every cell (int/rosenbrock.c:94-133): non-autonomous,
+ * Rodas4 unless the context was created with another method, tolerances
+ * set by kpp_ctx_set_tol (default ATOL 1, RTOL 1e-3 scalar), STEPMIN per
+ * cell carried from the previous call's Hexit. */
+int kpp_ctx_set_tol(kpp_ctx* ctx, const double* atol, const double* rtol, int vector_tol);
+int INTEGRATE_BATCH(kpp_ctx* ctx, double TIN, double TOUT);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,138 @@
+"""Build the solver library and its generated mechanism code.
+
+Steps (all outputs stay in-tree and are git-ignored build products):
+  1. compile the mechanism compiler ``kppgen`` (g++, C++17);
+  2. run it on each mechanism's KPP files -> ``kpp_amd/_gen/<root>_{hip,model}.h``
+     and ``<root>_stats.json``;
+  3. compile one HIP translation unit per mechanism plus the C ABI for gfx950
+     and link ``kpp_amd/lib/libkpphip.so``.
+``build_oracle()`` compiles the CPU oracle (test infrastructure) from the
+generated C model headers into ``oracle/build/``.
+
+The KPP mechanism sources are read from ``$KPP_HOME`` (default
+``/root/reference``), the KPP distribution whose ``models/`` directory holds
+``small_strato.*`` and ``saprc99.*``.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import json
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+REPO = PKG.parent
+CSRC = PKG / "csrc"
+GEN = PKG / "_gen"
+OBJ = PKG / "_obj"
+LIB = PKG / "lib"
+ORACLE = REPO / "oracle"
+ORACLE_BUILD = ORACLE / "build"
+
+# mechanism root -> KPP input file, relative to KPP_HOME
+MECHANISMS = {
+    "small_strato": "models/small_strato.def",
+    "saprc99": "models/saprc99.def",
+}
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # parity build: no FMA contraction, so device arithmetic is the oracle's
+    "-ffp-contract=off",
+    # packed FP32 instructions stay off in every kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def kpp_home() -> Path:
+    return Path(os.environ.get("KPP_HOME", "/root/reference"))
+
+
+def run(cmd, **kw):
+    r = subprocess.run(cmd, capture_output=True, text=True, **kw)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(map(str, cmd))}\n"
+                           f"{r.stdout[-4000:]}\n{r.stderr[-8000:]}")
+    return r
+
+
+def build_kppgen() -> Path:
+    OBJ.mkdir(parents=True, exist_ok=True)
+    exe = OBJ / "kppgen"
+    run(["g++", "-std=c++17", "-O2", "-Wall", "-Wno-unused-function", "-o", str(exe),
+         str(CSRC / "kppgen" / "kppgen.cpp")])
+    return exe
+
+
+def generate(mechs=None) -> list[str]:
+    """Run kppgen on the mechanisms whose sources are available."""
+    home = kpp_home()
+    exe = build_kppgen()
+    GEN.mkdir(parents=True, exist_ok=True)
+    done = []
+    for root, rel in (mechs or MECHANISMS).items():
+        src = home / rel
+        if not src.exists():
+            raise FileNotFoundError(f"KPP mechanism source {src} not found (set KPP_HOME)")
+        run([str(exe), "--kpp-home", str(home), "--root", root, "-o", str(GEN), str(src)])
+        (GEN / f"kpphip_{root}.hip").write_text(
+            f'#include "{root}_hip.h"\n#include "mech_launch.h"\nKPPHIP_DEFINE_MECH({root})\n')
+        done.append(root)
+    (GEN / "kpphip_registry.inc").write_text("".join(f"KPPHIP_MECH({r})\n" for r in done))
+    return done
+
+
+def _hipcc_obj(src: Path, out: Path):
+    run([HIPCC, *HIP_FLAGS, "-I", str(CSRC), "-I", str(GEN), "-I", str(REPO / "include"),
+         "-c", "-x", "hip", str(src), "-o", str(out)])
+    return out
+
+
+def build_lib(mechs: list[str], jobs: int = 8) -> Path:
+    OBJ.mkdir(parents=True, exist_ok=True)
+    LIB.mkdir(parents=True, exist_ok=True)
+    srcs = [GEN / f"kpphip_{m}.hip" for m in mechs] + [CSRC / "kpphip_api.cpp"]
+    objs = [OBJ / (s.stem + ".o") for s in srcs]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_hipcc_obj, srcs, objs))
+    out = LIB / "libkpphip.so"
+    tmp = LIB / "libkpphip.so.tmp"
+    run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle(mechs: list[str]) -> list[Path]:
+    """CPU oracle per mechanism: parity build (-O -ffp-contract=off, the flags
+    of util/Makefile_c:3 plus no contraction) and a timing build."""
+    ORACLE_BUILD.mkdir(parents=True, exist_ok=True)
+    outs = []
+    for m in mechs:
+        hdr = f'-DKPP_MODEL_HEADER="{m}_model.h"'
+        common = ["-fPIC", "-shared", "-fopenmp", "-Wall", "-Wno-unused-function", hdr,
+                  "-I", str(GEN), str(ORACLE / "ros_ref.c"), "-lm"]
+        parity = ORACLE_BUILD / f"libros_ref_{m}.so"
+        run(["gcc", "-O", "-ffp-contract=off", *common[:-2], "-o", str(parity), *common[-2:]])
+        fast = ORACLE_BUILD / f"libros_ref_{m}_fast.so"
+        run(["gcc", "-O2", "-march=x86-64-v3", *common[:-2], "-o", str(fast), *common[-2:]])
+        outs += [parity, fast]
+    return outs
+
+
+def build_all(with_oracle: bool = True) -> dict:
+    mechs = generate()
+    lib = build_lib(mechs)
+    info = {"lib": str(lib), "mechanisms": mechs}
+    if with_oracle:
+        info["oracle"] = [str(p) for p in build_oracle(mechs)]
+    return info
+
+
+if __name__ == "__main__":
+    print(json.dumps(build_all("--no-oracle" not in sys.argv), indent=1))

@@ -1,0 +1,1218 @@
+// kppgen — mechanism compiler for the MI355X batched Rosenbrock solver.
+//
+// Reads a mechanism written in the KPP language (.def/.spc/.eqn/.kpp files)
+// and emits, from the same expression trees:
+//   <root>_model.h  plain C model functions with KPP's C ABI semantics
+//                   (Fun, Jac_SP, KppSolve, Update_RCONST, LU pattern arrays);
+//                   consumed by the CPU oracle.
+//   <root>_hip.h    HIP __device__ versions of Fun, Jac_SP (fused with the
+//                   Ghimj build), an unrolled KppDecomp, KppSolve and the rate
+//                   constants, templated on an element accessor so the same
+//                   code runs on SoA global memory, LDS or register arrays.
+//   <root>_stats.json  sizes and exact operation counts of every routine.
+//
+// What has to match KPP (and why) — each point restates the cited reference:
+//  * species numbering: first occurrence in the equations
+//    (src/scanner.c:712-766 ProcessTerm), declaration order as the "code";
+//  * variable ordering: reactive first, then by code (src/kpp.c:73-88
+//    CodeRCmp, ReorderSpecies UNSORT :273-402), then Markowitz
+//    (src/kpp.c:220-271 BestSparsity, first strict minimum of (cnz-1)(lnz-1));
+//  * Jacobian and LU patterns (src/kpp.c:113-158, src/gen.c:331-356 NonZero);
+//  * stoichiometric coefficients stored as float and printed with "%g"
+//    (src/scanner.c:712-766, src/code_c.c:66), NUMBER rates folded in as
+//    "%g"-of-float constants (src/gen.c:118-125 RConst);
+//  * the simplification rules of the expression builder (src/code.c:519-597)
+//    and left-to-right association of the A[], Vdot[], B[], JVS[] and solve
+//    sums (src/gen.c:713-761, 1239-1284, 1991-2024), because the association
+//    order fixes the floating-point result.
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+[[noreturn]] void die(const std::string& m) {
+  fprintf(stderr, "kppgen: error: %s\n", m.c_str());
+  exit(2);
+}
+
+std::string upper(std::string s) {
+  for (auto& c : s) c = (char)toupper((unsigned char)c);
+  return s;
+}
+bool eq_nocase(const std::string& a, const std::string& b) { return upper(a) == upper(b); }
+std::string trim(const std::string& s) {
+  size_t b = 0, e = s.size();
+  while (b < e && isspace((unsigned char)s[b])) b++;
+  while (e > b && isspace((unsigned char)s[e - 1])) e--;
+  return s.substr(b, e - b);
+}
+bool file_exists(const std::string& p) {
+  std::ifstream f(p);
+  return f.good();
+}
+std::string read_file(const std::string& p) {
+  std::ifstream f(p, std::ios::binary);
+  if (!f) die("cannot open " + p);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  return ss.str();
+}
+std::string dirname_of(const std::string& p) {
+  auto k = p.find_last_of('/');
+  return k == std::string::npos ? std::string(".") : p.substr(0, k);
+}
+
+// ---------------------------------------------------------------------------
+// Lexical pass: comments, #INCLUDE, #INLINE, and splitting into command
+// sections.  Mirrors the scanner states of src/scan.l:112-296 for the subset
+// of commands the solver needs: '{...}' and '//' comments are dropped outside
+// inline code, a '#WORD' starts a command, inline code is kept verbatim up to
+// #ENDINLINE.
+// ---------------------------------------------------------------------------
+struct Section {
+  std::string cmd;   // upper-case command name, without '#'
+  std::string body;  // text up to the next command
+};
+
+struct Scan {
+  std::vector<std::string> search;  // include search directories
+  std::vector<Section> sections;
+  std::map<std::string, std::string> inlines;  // inline context -> code
+  int depth = 0;
+
+  std::string resolve(const std::string& name, const std::string& from_dir) {
+    std::vector<std::string> cands;
+    auto add = [&](const std::string& d) {
+      cands.push_back(d + "/" + name);
+      // KPP appends its default extension when the name has none
+      // (src/scanutil.c:142-200 FileName).
+      if (name.find('.') == std::string::npos) cands.push_back(d + "/" + name + ".kpp");
+    };
+    if (!name.empty() && name[0] == '/') {
+      cands.push_back(name);
+    } else {
+      add(from_dir);
+      for (auto& d : search) add(d);
+    }
+    for (auto& c : cands)
+      if (file_exists(c)) return c;
+    die("cannot resolve include '" + name + "' from " + from_dir);
+  }
+
+  static bool is_tok_char(char c) {  // STRING of src/scan.l:101: [^ \t\n{}#;]
+    return !(c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '{' || c == '}' || c == '#' ||
+             c == ';');
+  }
+
+  void file(const std::string& path) {
+    if (++depth > 16) die("include nesting too deep at " + path);
+    std::string t = read_file(path);
+    std::string dir = dirname_of(path);
+    size_t i = 0, n = t.size();
+    auto read_token = [&](size_t& k) {
+      while (k < n && (t[k] == ' ' || t[k] == '\t')) k++;
+      size_t b = k;
+      while (k < n && is_tok_char(t[k])) k++;
+      return t.substr(b, k - b);
+    };
+    if (sections.empty()) sections.push_back({"", ""});
+    while (i < n) {
+      char c = t[i];
+      if (c == '{') {
+        size_t j = t.find('}', i + 1);
+        i = (j == std::string::npos) ? n : j + 1;
+        continue;
+      }
+      if (c == '/' && i + 1 < n && t[i + 1] == '/') {
+        size_t j = t.find('\n', i);
+        i = (j == std::string::npos) ? n : j;
+        continue;
+      }
+      if (c == '#') {
+        size_t k = i + 1;
+        std::string w = upper(read_token(k));
+        if (w == "INLINE") {
+          std::string ctx = upper(read_token(k));
+          // code starts after the context name; ends at "#ENDINLINE"
+          size_t b = k, e = b;
+          for (;;) {
+            e = t.find('#', e);
+            if (e == std::string::npos) die("unterminated #INLINE " + ctx + " in " + path);
+            size_t q = e + 1;
+            std::string w2 = upper(read_token(q));
+            if (w2 == "ENDINLINE") {
+              inlines[ctx] += t.substr(b, e - b);
+              k = q;
+              break;
+            }
+            e = e + 1;
+          }
+          i = k;
+          continue;
+        }
+        if (w == "INCLUDE" || w == "MODEL") {
+          std::string f = read_token(k);
+          std::string p = (w == "MODEL") ? resolve(f + ".def", dir) : resolve(f, dir);
+          i = k;
+          file(p);
+          continue;
+        }
+        sections.push_back({w, ""});
+        i = k;
+        continue;
+      }
+      sections.back().body += c;
+      i++;
+    }
+    depth--;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Mechanism tables (src/gdata.h): species codes, stoichiometry, rates.
+// ---------------------------------------------------------------------------
+enum SpcType { VAR_SPC = 1, FIX_SPC = 2, DUMMY_SPC = 3 };
+enum RateType { NUMBER = 1, EXPRESSION = 2, PHOTO = 3 };
+
+struct SpeciesDef {
+  std::string name;
+  int type;
+  std::string ival;  // initial value text from #INITVALUES
+};
+
+struct Rate {
+  std::string label;
+  int type;
+  double f = 0;      // NUMBER value
+  std::string expr;  // EXPRESSION / PHOTO text (tokens concatenated)
+};
+
+constexpr int CODE_HV = 1000000 - 1;   // stands for MAX_SPECIES-1 (KPP's "HV")
+constexpr int CODE_PROD = 1000000 - 2;  // MAX_SPECIES-2 ("PROD")
+
+struct Mech {
+  std::string root;
+  std::vector<SpeciesDef> table;  // index = code; [0] is KPP's "???" dummy
+  SpeciesDef hv{"HV", DUMMY_SPC, ""}, prod{"PROD", DUMMY_SPC, ""};
+  double cfactor = 1.0;
+  std::string varDefault = "0", fixDefault = "0";
+  bool cfactor_set = false;
+
+  // per appearance-order species index ("crtSpec"): code, reactive flag
+  std::vector<int> code;
+  std::vector<int> reactive;
+  std::map<int, int> reverse;  // code -> appearance index
+  // stoichiometry [species appearance index][equation], float like KPP
+  std::vector<std::vector<float>> sl, sr, st;
+  std::vector<Rate> rates;
+
+  SpeciesDef& spc(int c) {
+    if (c == CODE_HV) return hv;
+    if (c == CODE_PROD) return prod;
+    return table[c];
+  }
+  int find_species(const std::string& n) {
+    for (size_t i = 0; i < table.size(); i++)
+      if (eq_nocase(table[i].name, n)) return (int)i;
+    if (eq_nocase(n, "HV")) return CODE_HV;
+    if (eq_nocase(n, "PROD")) return CODE_PROD;
+    return -1;
+  }
+};
+
+std::vector<std::string> split_semis(const std::string& body) {
+  std::vector<std::string> out;
+  std::string cur;
+  for (char c : body) {
+    if (c == ';') {
+      out.push_back(cur);
+      cur.clear();
+    } else {
+      cur += c;
+    }
+  }
+  if (!trim(cur).empty()) out.push_back(cur);
+  return out;
+}
+
+void declare_species(Mech& m, const std::string& body, int type) {
+  for (auto& item : split_semis(body)) {
+    std::string s = trim(item);
+    if (s.empty()) continue;
+    auto eqp = s.find('=');
+    std::string name = trim(eqp == std::string::npos ? s : s.substr(0, eqp));
+    if (name.empty()) continue;
+    if (m.find_species(name) >= 0) die("multiple declaration for species " + name);
+    m.table.push_back({name, type, ""});
+  }
+}
+
+void set_type(Mech& m, const std::string& body, int type) {  // #SETVAR / #SETFIX
+  for (auto& item : split_semis(body)) {
+    std::string n = trim(item);
+    if (n.empty()) continue;
+    if (eq_nocase(n, "VAR_SPEC") || eq_nocase(n, "FIX_SPEC") || eq_nocase(n, "ALL_SPEC")) {
+      int from = eq_nocase(n, "VAR_SPEC") ? VAR_SPC : eq_nocase(n, "FIX_SPEC") ? FIX_SPC : 0;
+      for (size_t i = 1; i < m.table.size(); i++)
+        if (!from || m.table[i].type == from) m.table[i].type = type;
+      continue;
+    }
+    int c = m.find_species(n);
+    if (c < 0) die("undefined species " + n);
+    m.spc(c).type = type;
+  }
+}
+
+void init_values(Mech& m, const std::string& body) {
+  for (auto& item : split_semis(body)) {
+    std::string s = trim(item);
+    if (s.empty()) continue;
+    auto eqp = s.find('=');
+    if (eqp == std::string::npos) die("bad #INITVALUES entry: " + s);
+    std::string n = trim(s.substr(0, eqp)), v = trim(s.substr(eqp + 1));
+    if (eq_nocase(n, "CFACTOR")) {
+      m.cfactor = strtod(v.c_str(), nullptr);
+      m.cfactor_set = true;
+    } else if (eq_nocase(n, "VAR_SPEC")) {
+      m.varDefault = v;
+    } else if (eq_nocase(n, "FIX_SPEC")) {
+      m.fixDefault = v;
+    } else if (eq_nocase(n, "ALL_SPEC")) {
+      m.varDefault = m.fixDefault = v;
+    } else {
+      int c = m.find_species(n);
+      if (c < 0) die("undefined species in #INITVALUES: " + n);
+      m.spc(c).ival = v;
+    }
+  }
+}
+
+// One equation "<tag> lhs = rhs : rate" (src/scan.y:307-359).
+void parse_equation(Mech& m, const std::string& text) {
+  std::string s = trim(text);
+  if (s.empty()) return;
+  std::string label = "          ";
+  size_t i = 0;
+  if (s[0] == '<') {
+    auto k = s.find('>');
+    if (k == std::string::npos) die("bad equation tag: " + s);
+    label = trim(s.substr(1, k - 1));
+    i = k + 1;
+  }
+  auto eqp = s.find('=', i);
+  auto colon = s.find(':', i);
+  if (eqp == std::string::npos || colon == std::string::npos || colon < eqp)
+    die("bad equation: " + s);
+  std::string lhs = s.substr(i, eqp - i), rhs = s.substr(eqp + 1, colon - eqp - 1);
+  std::string rate_raw = s.substr(colon + 1);
+  int eqn = (int)m.rates.size();
+  for (auto& row : m.sl) row.push_back(0.f);
+  for (auto& row : m.sr) row.push_back(0.f);
+  for (auto& row : m.st) row.push_back(0.f);
+  bool is_photo = false;
+
+  auto process = [&](int side, const std::string& sign, const std::string& coef,
+                     const std::string& name) {
+    int c = m.find_species(name);
+    if (c < 0) die("undefined species '" + name + "' in equation <" + label + ">");
+    if (eq_nocase(name, "HV")) is_photo = true;
+    int k;
+    auto it = m.reverse.find(c);
+    if (it == m.reverse.end()) {
+      k = (int)m.code.size();
+      m.code.push_back(c);
+      m.reactive.push_back(0);
+      m.reverse[c] = k;
+      m.sl.emplace_back(eqn + 1, 0.f);
+      m.sr.emplace_back(eqn + 1, 0.f);
+      m.st.emplace_back(eqn + 1, 0.f);
+    } else {
+      k = it->second;
+    }
+    double val = strtod((sign + coef).c_str(), nullptr);
+    if (side == 0) {
+      m.sl[k][eqn] += val;
+      m.st[k][eqn] -= val;
+      m.reactive[k] = 1;
+    } else {
+      m.sr[k][eqn] += val;
+      m.st[k][eqn] += val;
+    }
+  };
+  auto terms = [&](const std::string& e, int side) {
+    size_t p = 0, n = e.size();
+    std::string sign = "+";
+    while (p < n) {
+      char c = e[p];
+      if (isspace((unsigned char)c)) { p++; continue; }
+      if (c == '+' || c == '-') { sign = std::string(1, c); p++; continue; }
+      std::string coef = "1";
+      if (isdigit((unsigned char)c) || c == '.') {  // UFLOAT of src/scan.l:112
+        size_t b = p;
+        while (p < n && (isdigit((unsigned char)e[p]) || e[p] == '.')) p++;
+        if (p < n && (e[p] == 'e' || e[p] == 'E' || e[p] == 'd' || e[p] == 'D') && p + 1 < n &&
+            (isdigit((unsigned char)e[p + 1]) ||
+             ((e[p + 1] == '+' || e[p + 1] == '-') && p + 2 < n && isdigit((unsigned char)e[p + 2])))) {
+          p += 2;
+          while (p < n && isdigit((unsigned char)e[p])) p++;
+        }
+        coef = e.substr(b, p - b);
+        while (p < n && isspace((unsigned char)e[p])) p++;
+      }
+      if (p < n && (isalpha((unsigned char)e[p]) || e[p] == '_')) {
+        size_t b = p;
+        while (p < n && (isalnum((unsigned char)e[p]) || e[p] == '_')) p++;
+        process(side, sign, coef, e.substr(b, p - b));
+        sign = "+";
+        continue;
+      }
+      die("cannot parse equation term in <" + label + ">: '" + e + "'");
+    }
+  };
+  terms(lhs, 0);
+  terms(rhs, 1);
+  // Rate text: whitespace-separated tokens concatenated (src/scan.y:324-330).
+  std::string rate;
+  for (char c : rate_raw)
+    if (!isspace((unsigned char)c)) rate += c;
+  Rate r;
+  r.label = label;
+  if (is_photo) {
+    r.type = PHOTO;
+    r.expr = rate;
+  } else {
+    // NUMBER iff the whole text is one floating literal (src/scanner.c:604-628)
+    char* end = nullptr;
+    double f = strtod(rate.c_str(), &end);
+    if (end != rate.c_str() && *end == 0) {
+      r.type = NUMBER;
+      r.f = f;
+    } else {
+      r.type = EXPRESSION;
+      r.expr = rate;
+    }
+  }
+  m.rates.push_back(r);
+}
+
+// ---------------------------------------------------------------------------
+// Structural analysis (src/kpp.c).  After reorder(), species index k has
+// code[k], stoichiometry sl/sr/st[k][*]; [0,nvar) are variable, then fixed.
+// ---------------------------------------------------------------------------
+struct Structure {
+  int nvar = 0, nfix = 0, nspc = 0, nreact = 0;
+  std::vector<std::vector<int>> J, LU;  // nvar x nvar
+  std::vector<int> lu_icol, lu_irow, lu_crow, lu_diag;
+  int nonzero = 0, lu_nonzero = 0;
+};
+
+void struct_j(const Mech& m, int nvar, std::vector<std::vector<int>>& J) {
+  J.assign(nvar, std::vector<int>(nvar, 0));
+  int ne = (int)m.rates.size();
+  for (int i = 0; i < nvar; i++) J[i][i] = 1;
+  for (int i = 0; i < nvar; i++)
+    for (int j = 0; j < nvar; j++)
+      for (int k = 0; k < ne; k++)
+        if (m.st[i][k] * (m.sl[j][k] != 0 ? 1 : 0) != 0.0f) {
+          J[i][j] = 1;
+          break;
+        }
+}
+
+void lu_fill(std::vector<std::vector<int>>& L, int n) {  // src/kpp.c:133-158
+  for (int j = 0; j < n - 1; j++)
+    for (int i = j + 1; i < n; i++)
+      if (L[i][j])
+        for (int k = j; k < n; k++)
+          if (L[j][k] != 0) L[i][k] = 1;
+}
+
+// Rebuild the per-index tables in a new species order given as a list of
+// appearance indices (variables first, then fixed, then dummies).
+void permute(Mech& m, const std::vector<int>& order) {
+  auto P = [&](auto& v) {
+    auto old = v;
+    for (size_t k = 0; k < order.size(); k++) v[k] = old[order[k]];
+  };
+  P(m.code);
+  P(m.reactive);
+  P(m.sl);
+  P(m.sr);
+  P(m.st);
+  m.reverse.clear();
+  for (size_t k = 0; k < m.code.size(); k++) m.reverse[m.code[k]] = (int)k;
+}
+
+Structure reorder(Mech& m, bool use_reorder) {
+  Structure S;
+  int n = (int)m.code.size();
+  std::vector<int> var, fix, dummy;
+  for (int k = 0; k < n; k++) {
+    int t = m.spc(m.code[k]).type;
+    if (t == VAR_SPC) var.push_back(k);
+    else if (t == FIX_SPC) fix.push_back(k);
+    else dummy.push_back(k);
+  }
+  // UNSORT: reactive variables first, each group by code (src/kpp.c:73-88);
+  // fixed species by code.
+  std::sort(var.begin(), var.end(), [&](int a, int b) {
+    if (m.reactive[a] != m.reactive[b]) return m.reactive[a] > m.reactive[b];
+    return m.code[a] < m.code[b];
+  });
+  std::sort(fix.begin(), fix.end(), [&](int a, int b) { return m.code[a] < m.code[b]; });
+  std::vector<int> order;
+  order.insert(order.end(), var.begin(), var.end());
+  order.insert(order.end(), fix.begin(), fix.end());
+  order.insert(order.end(), dummy.begin(), dummy.end());
+  permute(m, order);
+  S.nvar = (int)var.size();
+  S.nfix = (int)fix.size();
+  S.nspc = S.nvar + S.nfix;
+  S.nreact = (int)m.rates.size();
+
+  if (use_reorder && S.nvar > 0) {
+    // BestSparsity (src/kpp.c:220-271): greedy Markowitz on the symbolic
+    // structure with explicit fill, first strict minimum wins.
+    std::vector<std::vector<int>> L;
+    struct_j(m, S.nvar, L);
+    int nv = S.nvar;
+    std::vector<int> best(nv);
+    for (int i = 0; i < nv; i++) best[i] = i;
+    for (int s = 0; s < nv - 1; s++) {
+      long bestv = 1L << 62;
+      int bi = 0;
+      for (int i = s; i < nv; i++) {
+        long cnz = 0, lnz = 0;
+        for (int j = s; j < nv; j++) {
+          cnz += L[i][j] ? 1 : 0;
+          lnz += L[j][i] ? 1 : 0;
+        }
+        long crt = (cnz - 1) * (lnz - 1);
+        if (crt < bestv) {
+          bestv = crt;
+          bi = i;
+        }
+      }
+      std::swap(L[s], L[bi]);
+      for (int i = 0; i < nv; i++) std::swap(L[i][s], L[i][bi]);
+      std::swap(best[s], best[bi]);
+      for (int i = s + 1; i < nv; i++)
+        if (L[i][s])
+          for (int k = s; k < nv; k++) L[i][k] += L[s][k];
+    }
+    std::vector<int> ord2;
+    for (int i = 0; i < nv; i++) ord2.push_back(best[i]);
+    for (int k = nv; k < n; k++) ord2.push_back(k);
+    permute(m, ord2);
+  }
+  struct_j(m, S.nvar, S.J);
+  S.LU = S.J;
+  lu_fill(S.LU, S.nvar);
+  // NonZero (src/gen.c:331-356), row-major, diagonal always present.
+  auto nonzero = [&](const std::vector<std::vector<int>>& M, std::vector<int>* irow,
+                     std::vector<int>* icol, std::vector<int>* crow, std::vector<int>* diag) {
+    int ne = 0;
+    for (int i = 0; i < S.nvar; i++) {
+      if (crow) crow->push_back(ne);
+      for (int j = 0; j < S.nvar; j++) {
+        if (i == j || M[i][j]) {
+          if (irow) irow->push_back(i);
+          if (icol) icol->push_back(j);
+          ne++;
+        }
+        if (i == j && diag) diag->push_back(ne - 1);
+      }
+    }
+    if (crow) crow->push_back(ne);
+    if (diag) diag->push_back(ne);
+    return ne;
+  };
+  S.nonzero = nonzero(S.J, nullptr, nullptr, nullptr, nullptr);
+  S.lu_nonzero = nonzero(S.LU, &S.lu_irow, &S.lu_icol, &S.lu_crow, &S.lu_diag);
+  return S;
+}
+
+// ---------------------------------------------------------------------------
+// Expression nodes with the simplification rules of src/code.c:519-597.
+// A node is a constant (|value| as float plus a sign), an element reference,
+// or a binary operation.  Printing hoists leaf signs: in IEEE round-to-nearest
+// a + (-b) == a - b and (-a)*b == -(a*b) exactly, so only the association of
+// the tree decides the result, and the tree is kept as KPP builds it.
+// ---------------------------------------------------------------------------
+struct Node {
+  enum K { CONST, ELM, ADD, SUB, MUL, DIV } k;
+  int sign = 1;
+  float cval = 0;     // CONST: absolute value
+  std::string ref;    // ELM: printed reference, e.g. "V[3]"
+  Node *l = nullptr, *r = nullptr;
+};
+std::vector<Node*> g_nodes;
+Node* mk(Node::K k) {
+  Node* n = new Node();
+  n->k = k;
+  g_nodes.push_back(n);
+  return n;
+}
+Node* Const(double v) {
+  Node* n = mk(Node::CONST);
+  n->cval = (float)v;
+  if (n->cval < 0) {
+    n->cval = -n->cval;
+    n->sign = -1;
+  }
+  return n;
+}
+Node* Ref(const std::string& s) {
+  Node* n = mk(Node::ELM);
+  n->ref = s;
+  return n;
+}
+bool is_const(Node* n, float v) { return n && n->k == Node::CONST && n->cval == v; }
+Node* bin(Node::K k, Node* a, Node* b) {
+  Node* n = mk(k);
+  n->l = a;
+  n->r = b;
+  return n;
+}
+Node* Add(Node* a, Node* b) {
+  if (!a) return b;
+  if (!b) return a;
+  if (is_const(a, 0)) return b;
+  if (is_const(b, 0)) return a;
+  return bin(Node::ADD, a, b);
+}
+Node* Sub(Node* a, Node* b) {
+  if (!a) return bin(Node::SUB, nullptr, b);
+  if (!b) return a;
+  if (is_const(a, 0)) return bin(Node::SUB, nullptr, b);
+  if (is_const(b, 0)) return a;
+  return bin(Node::SUB, a, b);
+}
+Node* Mul(Node* a, Node* b) {
+  if (!a) return b;
+  if (!b) return a;
+  if (is_const(a, 1)) {  // the constant's sign moves to the other operand
+    b->sign *= a->sign;
+    return b;
+  }
+  if (is_const(b, 1)) return a;  // KPP drops the sign of a right unit constant
+  if (is_const(a, 0)) return a;
+  if (is_const(b, 0)) return b;
+  return bin(Node::MUL, a, b);
+}
+Node* Div(Node* a, Node* b) {
+  if (!a) return bin(Node::DIV, Const(1), b);
+  if (!b) return a;
+  if (is_const(b, 1)) return a;
+  return bin(Node::DIV, a, b);
+}
+
+std::string cfmt(float v) {  // "%g" of the float, as src/code_c.c:66
+  char b[64];
+  snprintf(b, sizeof b, "%g", (double)v);
+  std::string s = b;
+  // make it a double literal in C/HIP without changing its value
+  if (s.find_first_of(".eEn") == std::string::npos) s += ".0";
+  return s;
+}
+
+// CFACTOR is written with "%12.6e" by KPP (src/gen.c:104-115 ascid).
+std::string efmt(double v) {
+  char b[64];
+  snprintf(b, sizeof b, "%.6e", v);
+  return b;
+}
+
+// Operation counter for the emitted arithmetic.
+struct Ops {
+  long add = 0, mul = 0, div = 0;
+  long total() const { return add + mul + div; }
+};
+
+// Print a node. Leaf signs are folded into the enclosing +/- or into a
+// leading unary minus; product/quotient chains print left-deep, which C
+// evaluates left to right.
+struct Printer {
+  Ops* ops;
+  int prec(Node* n) {
+    switch (n->k) {
+      case Node::ADD: case Node::SUB: return 1;
+      case Node::MUL: case Node::DIV: return 2;
+      default: return 3;
+    }
+  }
+  std::string leaf(Node* n) { return n->k == Node::CONST ? cfmt(n->cval) : n->ref; }
+  bool is_leaf(Node* n) { return n->k == Node::CONST || n->k == Node::ELM; }
+  bool is_muldiv(Node* n) { return n->k == Node::MUL || n->k == Node::DIV; }
+  // sign of a multiplicative chain: leaf signs hoisted out of the chain.
+  // Signs stored on MUL/ADD nodes themselves are not printed by KPP
+  // (src/code.c:692-764 writes leaf signs only), so they are ignored here.
+  int chain_sign(Node* n) {
+    if (is_leaf(n)) return n->sign;
+    if (is_muldiv(n))
+      return ((is_leaf(n->l) || is_muldiv(n->l)) ? chain_sign(n->l) : 1) *
+             (is_leaf(n->r) ? n->r->sign : 1);
+    return 1;
+  }
+  std::string unsigned_mul(Node* n) {  // product/quotient chain without signs
+    if (is_leaf(n)) return leaf(n);
+    std::string L = (is_leaf(n->l) || is_muldiv(n->l)) ? unsigned_mul(n->l) : "(" + expr(n->l) + ")";
+    std::string R = is_leaf(n->r) ? leaf(n->r) : "(" + expr(n->r) + ")";
+    if (n->k == Node::MUL) ops->mul++; else ops->div++;
+    return L + (n->k == Node::MUL ? "*" : "/") + R;
+  }
+  // a term of a sum: returns (sign, text)
+  std::pair<int, std::string> term(Node* n) {
+    if (n->k == Node::ADD || n->k == Node::SUB) return {1, "(" + expr(n) + ")"};
+    return {chain_sign(n), unsigned_mul(n)};
+  }
+  std::string expr(Node* n) {
+    if (!n) return "0.0";
+    if (n->k == Node::ADD || n->k == Node::SUB) {
+      std::string L;
+      if (n->l) {
+        if (n->l->k == Node::ADD || n->l->k == Node::SUB) {
+          L = expr(n->l);
+        } else {
+          auto t = term(n->l);
+          L = (t.first < 0 ? "-" : "") + t.second;
+        }
+      }
+      auto t = term(n->r);
+      int s = t.first * (n->k == Node::SUB ? -1 : 1);
+      if (n->l) ops->add++;
+      if (!n->l) return (s < 0 ? "-" : "") + t.second;
+      return L + (s < 0 ? "-" : "+") + t.second;
+    }
+    auto t = term(n);
+    return (t.first < 0 ? "-" : "") + t.second;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Model code construction shared by both emitters.
+// ---------------------------------------------------------------------------
+struct Model {
+  Mech* m;
+  Structure S;
+  // A[j] for used reactions
+  std::vector<int> a_used;
+  std::vector<std::string> a_expr;
+  std::vector<std::string> vdot_expr;
+  // B terms: (eqn i, var j) -> name, expression
+  std::map<std::pair<int, int>, int> b_index;
+  std::vector<std::string> b_expr;
+  std::vector<std::string> jvs_expr;  // per LU entry
+  std::vector<char> jvs_zero;
+  Ops fun_ops, jac_ops, lu_ops, sol_ops;
+  std::vector<std::pair<int, std::string>> rconst;  // (reaction, C expression)
+
+  Node* rconst_node(int j) {
+    auto& r = m->rates[j];
+    if (r.type == NUMBER) return Const(r.f);
+    return Ref("RCT[" + std::to_string(j) + "]");
+  }
+
+  void build() {
+    int nv = S.nvar, ns = S.nspc, ne = S.nreact;
+    Printer pf{&fun_ops};
+    // Fun (src/gen.c:713-761): A[j] for reactions touching a variable.
+    for (int j = 0; j < ne; j++) {
+      bool used = false;
+      for (int i = 0; i < nv; i++)
+        if (m->st[i][j] != 0) { used = true; break; }
+      if (!used) continue;
+      Node* p = rconst_node(j);
+      int i = 0;
+      for (; i < nv; i++)
+        for (int k = 1; k <= (int)m->sl[i][j]; k++) p = Mul(p, Ref("V[" + std::to_string(i) + "]"));
+      for (; i < ns; i++)
+        for (int k = 1; k <= (int)m->sl[i][j]; k++)
+          p = Mul(p, Ref("F[" + std::to_string(i - nv) + "]"));
+      a_used.push_back(j);
+      a_expr.push_back(pf.expr(p));
+    }
+    for (int i = 0; i < nv; i++) {
+      Node* sum = Const(0);
+      for (int j = 0; j < ne; j++)
+        sum = Add(sum, Mul(Const(m->st[i][j]), Ref("A[" + std::to_string(j) + "]")));
+      vdot_expr.push_back(pf.expr(sum));
+    }
+    // Jac_SP (src/gen.c:1239-1284)
+    Printer pj{&jac_ops};
+    for (int i = 0; i < ne; i++)
+      for (int j = 0; j < nv; j++) {
+        if (m->sl[j][i] == 0) continue;
+        Node* p = Mul(rconst_node(i), Const(m->sl[j][i]));
+        int l = 0;
+        for (; l < nv; l++) {
+          int mm = (int)m->sl[l][i] - (l == j);
+          for (int k = 1; k <= mm; k++) p = Mul(p, Ref("V[" + std::to_string(l) + "]"));
+        }
+        for (; l < ns; l++)
+          for (int k = 1; k <= (int)m->sl[l][i]; k++)
+            p = Mul(p, Ref("F[" + std::to_string(l - nv) + "]"));
+        b_index[{i, j}] = (int)b_expr.size();
+        b_expr.push_back(pj.expr(p));
+      }
+    for (int i = 0; i < nv; i++)
+      for (int j = 0; j < nv; j++) {
+        if (!S.LU[i][j]) continue;
+        Node* sum = Const(0);
+        for (int k = 0; k < ne; k++) {
+          if (m->st[i][k] != 0 && m->sl[j][k] != 0)
+            sum = Add(sum, Mul(Const(m->st[i][k]),
+                               Ref("B[" + std::to_string(b_index[{k, j}]) + "]")));
+        }
+        bool z = is_const(sum, 0);
+        jvs_zero.push_back(z);
+        jvs_expr.push_back(z ? std::string("0.0") : pj.expr(sum));
+      }
+    // op counts of the LU (src/util sutil.c:2-23 arithmetic) and the solve
+    for (int k = 0; k < nv; k++)
+      for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
+        int j = S.lu_icol[kk];
+        lu_ops.div++;
+        for (int jj = S.lu_diag[j] + 1; jj < S.lu_crow[j + 1]; jj++) {
+          lu_ops.mul++;
+          lu_ops.add++;
+        }
+      }
+    for (int i = 0; i < nv; i++) {
+      sol_ops.mul += (S.lu_diag[i] - S.lu_crow[i]) + (S.lu_crow[i + 1] - S.lu_diag[i] - 1);
+      sol_ops.add += (S.lu_diag[i] - S.lu_crow[i]) + (S.lu_crow[i + 1] - S.lu_diag[i] - 1);
+      sol_ops.div += 1;
+    }
+    for (int j = 0; j < ne; j++)
+      if (m->rates[j].type != NUMBER) rconst.push_back({j, m->rates[j].expr});
+  }
+};
+
+bool mentions(const std::string& expr, const std::string& ident) {
+  size_t p = 0;
+  while ((p = expr.find(ident, p)) != std::string::npos) {
+    bool lb = (p == 0) || !(isalnum((unsigned char)expr[p - 1]) || expr[p - 1] == '_');
+    size_t e = p + ident.size();
+    bool rb = (e >= expr.size()) || !(isalnum((unsigned char)expr[e]) || expr[e] == '_');
+    if (lb && rb) return true;
+    p = e;
+  }
+  return false;
+}
+
+// Rate laws of util/UserRateLaws.c:9-78 (parameters are float, arithmetic
+// double), written out for the emitted code.  The identifiers TEMP and
+// CFACTOR are the enclosing function's per-cell values.
+const char* kRateLawsBody = R"(
+KPPQ double kpp_ARR_abc(float A0, float B0, float C0, double TEMP) {
+  return (double)A0 * exp(-(double)B0 / TEMP) * pow((TEMP / 300.0), (double)C0);
+}
+KPPQ double kpp_ARR_ab(float A0, float B0, double TEMP) {
+  return (double)A0 * exp(-(double)B0 / TEMP);
+}
+KPPQ double kpp_ARR_ac(float A0, float C0, double TEMP) {
+  return (double)A0 * pow((TEMP / 300.0), (double)C0);
+}
+KPPQ double kpp_EP2(float A0, float C0, float A2, float C2, float A3, float C3, double TEMP,
+                    double CFACTOR) {
+  double K0, K2, K3;
+  K0 = (double)A0 * exp(-(double)C0 / TEMP);
+  K2 = (double)A2 * exp(-(double)C2 / TEMP);
+  K3 = (double)A3 * exp(-(double)C3 / TEMP);
+  K3 = K3 * CFACTOR * 1.0e+6;
+  return K0 + K3 / (1.0 + K3 / K2);
+}
+KPPQ double kpp_EP3(float A1, float C1, float A2, float C2, double TEMP, double CFACTOR) {
+  double K1, K2;
+  K1 = (double)A1 * exp(-(double)C1 / TEMP);
+  K2 = (double)A2 * exp(-(double)C2 / TEMP);
+  return K1 + K2 * (1.0e+6 * CFACTOR);
+}
+KPPQ double kpp_FALL(float A0, float B0, float C0, float A1, float B1, float C1, float CF,
+                     double TEMP, double CFACTOR) {
+  double K0, K1;
+  K0 = (double)A0 * exp(-(double)B0 / TEMP) * pow((TEMP / 300.0), (double)C0);
+  K1 = (double)A1 * exp(-(double)B1 / TEMP) * pow((TEMP / 300.0), (double)C1);
+  K0 = K0 * CFACTOR * 1.0e+6;
+  K1 = K0 / K1;
+  return (K0 / (1.0 + K1)) * pow((double)CF, (1.0 / (1.0 + pow((log10(K1)), 2))));
+}
+#define ARR_abc(a, b, c) kpp_ARR_abc((a), (b), (c), TEMP)
+#define ARR_ab(a, b) kpp_ARR_ab((a), (b), TEMP)
+#define ARR_ac(a, c) kpp_ARR_ac((a), (c), TEMP)
+#define EP2(a0, c0, a2, c2, a3, c3) kpp_EP2((a0), (c0), (a2), (c2), (a3), (c3), TEMP, CFACTOR)
+#define EP3(a1, c1, a2, c2) kpp_EP3((a1), (c1), (a2), (c2), TEMP, CFACTOR)
+#define FALL(a0, b0, c0, a1, b1, c1, cf) \
+  kpp_FALL((a0), (b0), (c0), (a1), (b1), (c1), (cf), TEMP, CFACTOR)
+)";
+
+std::string ident_ok(const std::string& root) {
+  std::string s;
+  for (char c : root) s += (isalnum((unsigned char)c) ? c : '_');
+  return s;
+}
+
+void emit_int_array(std::ostream& o, const char* qual, const std::string& name,
+                    const std::vector<int>& v) {
+  o << qual << " int " << name << "[" << std::max<size_t>(v.size(), 1) << "] = {";
+  for (size_t i = 0; i < v.size(); i++) {
+    if (i % 16 == 0) o << "\n  ";
+    o << v[i] << (i + 1 < v.size() ? "," : "");
+  }
+  if (v.empty()) o << "0";
+  o << "};\n";
+}
+
+// initial value of each species in the final order, or the group default
+std::vector<std::string> init_texts(Model& md) {
+  std::vector<std::string> out;
+  for (int k = 0; k < md.S.nspc; k++) {
+    auto& sp = md.m->spc(md.m->code[k]);
+    std::string v = sp.ival;
+    if (v.empty()) v = (k < md.S.nvar) ? md.m->varDefault : md.m->fixDefault;
+    out.push_back(v);
+  }
+  return out;
+}
+
+void emit_c_model(Model& md, const std::string& path) {
+  Mech& m = *md.m;
+  auto& S = md.S;
+  std::ofstream o(path);
+  o << "/* Generated by kppgen from the KPP mechanism '" << m.root << "'.\n"
+    << "   Plain C model functions with the semantics of KPP's C output\n"
+    << "   (src/gen.c GenerateFun/GenerateJac/GenerateSolve/GenerateUpdateRconst). */\n";
+  o << "#ifndef KPPGEN_" << ident_ok(m.root) << "_MODEL_H\n#define KPPGEN_" << ident_ok(m.root)
+    << "_MODEL_H\n#include <math.h>\n\n";
+  o << "#define KPP_ROOT \"" << m.root << "\"\n";
+  o << "#define KPP_NVAR " << S.nvar << "\n#define KPP_NFIX " << S.nfix << "\n#define KPP_NSPEC "
+    << S.nspc << "\n#define KPP_NREACT " << S.nreact << "\n#define KPP_NONZERO " << S.nonzero
+    << "\n#define KPP_LU_NONZERO " << S.lu_nonzero << "\n#define KPP_CFACTOR " << efmt(m.cfactor)
+    << "\n\n";
+  emit_int_array(o, "static const", "kpp_LU_ICOL", S.lu_icol);
+  emit_int_array(o, "static const", "kpp_LU_IROW", S.lu_irow);
+  emit_int_array(o, "static const", "kpp_LU_CROW", S.lu_crow);
+  emit_int_array(o, "static const", "kpp_LU_DIAG", S.lu_diag);
+  o << "static const char* const kpp_SPC_NAMES[" << S.nspc << "] = {";
+  for (int k = 0; k < S.nspc; k++) o << (k % 8 ? " " : "\n  ") << "\"" << m.spc(m.code[k]).name << "\",";
+  o << "};\n";
+  auto iv = init_texts(md);
+  o << "/* #INITVALUES text times CFACTOR, as Initialize() (src/gen.c:3252-3290) */\n";
+  o << "static void kpp_Initialize(double* C) {\n  const double CFACTOR = " << efmt(m.cfactor)
+    << ";\n";
+  for (int k = 0; k < S.nspc; k++) o << "  C[" << k << "] = (" << iv[k] << ")*CFACTOR;\n";
+  o << "}\n";
+  o << "#define KPPQ static\n" << kRateLawsBody << "#undef KPPQ\n";
+  o << "/* Update_RCONST (src/gen.c:2343-2473); NUMBER rates are folded into\n"
+    << "   Fun/Jac_SP as constants, so their RCONST slot is only informative. */\n";
+  o << "static void kpp_Update_RCONST(double TEMP, double SUN, double CFACTOR, double TIME,\n"
+    << "                              double* RCONST) {\n  (void)SUN; (void)TIME; (void)TEMP; "
+       "(void)CFACTOR;\n";
+  for (int j = 0; j < S.nreact; j++) {
+    auto& r = m.rates[j];
+    if (r.type == NUMBER) o << "  RCONST[" << j << "] = " << cfmt((float)r.f) << ";\n";
+    else o << "  RCONST[" << j << "] = (" << r.expr << ");\n";
+  }
+  o << "}\n";
+  o << "static void kpp_Fun(const double* V, const double* F, const double* RCT, double* "
+       "Vdot) {\n  double A[" << std::max(S.nreact, 1) << "];\n  (void)F; (void)RCT;\n";
+  for (size_t q = 0; q < md.a_used.size(); q++)
+    o << "  A[" << md.a_used[q] << "] = " << md.a_expr[q] << ";\n";
+  for (int i = 0; i < S.nvar; i++) o << "  Vdot[" << i << "] = " << md.vdot_expr[i] << ";\n";
+  o << "}\n";
+  o << "static void kpp_Jac_SP(const double* V, const double* F, const double* RCT, double* "
+       "JVS) {\n  double B[" << std::max<size_t>(md.b_expr.size(), 1)
+    << "];\n  (void)V; (void)F; (void)RCT;\n";
+  for (size_t q = 0; q < md.b_expr.size(); q++) o << "  B[" << q << "] = " << md.b_expr[q] << ";\n";
+  for (int n = 0; n < S.lu_nonzero; n++) o << "  JVS[" << n << "] = " << md.jvs_expr[n] << ";\n";
+  o << "}\n";
+  // KppSolve (src/gen.c:1960-2033)
+  o << "static void kpp_KppSolve(const double* JVS, double* X) {\n";
+  for (int i = 0; i < S.nvar; i++) {
+    int b = S.lu_crow[i], e = S.lu_diag[i];
+    if (b < e) {
+      o << "  X[" << i << "] = X[" << i << "]";
+      for (int j = b; j < e; j++) o << "-JVS[" << j << "]*X[" << S.lu_icol[j] << "]";
+      o << ";\n";
+    }
+  }
+  for (int i = S.nvar - 1; i >= 0; i--) {
+    int b = S.lu_diag[i] + 1, e = S.lu_crow[i + 1];
+    if (b < e) {
+      o << "  X[" << i << "] = (X[" << i << "]";
+      for (int j = b; j < e; j++) o << "-JVS[" << j << "]*X[" << S.lu_icol[j] << "]";
+      o << ")/JVS[" << S.lu_diag[i] << "];\n";
+    } else {
+      o << "  X[" << i << "] = X[" << i << "]/JVS[" << S.lu_diag[i] << "];\n";
+    }
+  }
+  o << "}\n";
+  o << "#undef ARR_abc\n#undef ARR_ab\n#undef ARR_ac\n#undef EP2\n#undef EP3\n#undef FALL\n";
+  o << "#endif\n";
+}
+
+// The HIP header: everything is a template over an element accessor, so the
+// same straight-line code runs on SoA global memory (p[i*stride]), LDS or
+// register arrays.  All indices are compile-time constants.
+void emit_hip(Model& md, const std::string& path) {
+  Mech& m = *md.m;
+  auto& S = md.S;
+  std::string ns = "kppm_" + ident_ok(m.root);
+  std::ofstream o(path);
+  o << "// Generated by kppgen from the KPP mechanism '" << m.root << "'.\n"
+    << "// HIP device functions of the model; arithmetic and association order are\n"
+    << "// those of KPP's C output, so with FP contraction off the results are\n"
+    << "// bitwise those of the C model.\n#pragma once\n#include <hip/hip_runtime.h>\n\n";
+  o << "struct " << ns << " {\n";
+  o << "static constexpr int NVAR = " << S.nvar << ", NFIX = " << S.nfix << ", NSPEC = " << S.nspc
+    << ", NREACT = " << S.nreact << ", NONZERO = " << S.nonzero
+    << ", LU_NONZERO = " << S.lu_nonzero << ";\n";
+  o << "static constexpr const char* ROOT = \"" << m.root << "\";\n";
+  // rate dependence classes
+  bool any_time = false;
+  std::vector<int> rc_const, rc_sun;
+  for (auto& pr : md.rconst) {
+    bool dep = mentions(pr.second, "SUN") || mentions(pr.second, "TIME");
+    if (mentions(pr.second, "TIME")) any_time = true;
+    (dep ? rc_sun : rc_const).push_back(pr.first);
+  }
+  o << "static constexpr bool RATES_USE_TIME = " << (any_time ? "true" : "false") << ";\n";
+  o << "static constexpr int N_SUN_RATES = " << rc_sun.size() << ";\n";
+  o << "#define KPPQ static __device__ __forceinline__\n" << kRateLawsBody << "#undef KPPQ\n";
+  o << "// rate constants that do not depend on SUN or TIME (once per cell)\n";
+  o << "template <class R> static __device__ __forceinline__ void rates_fixed(double TEMP, double "
+       "CFACTOR, R RCT) {\n  (void)TEMP; (void)CFACTOR;\n";
+  for (int j : rc_const) o << "  RCT[" << j << "] = (" << m.rates[j].expr << ");\n";
+  o << "}\n// rate constants that depend on SUN or TIME (every Fun/Jac call)\n";
+  o << "template <class R> static __device__ __forceinline__ void rates_sun(double TEMP, double "
+       "CFACTOR, double SUN, double TIME, R RCT) {\n  (void)TEMP; (void)CFACTOR; (void)SUN; "
+       "(void)TIME;\n";
+  for (int j : rc_sun) o << "  RCT[" << j << "] = (" << m.rates[j].expr << ");\n";
+  o << "}\n";
+  o << "template <class TV, class TF, class TR, class TO>\nstatic __device__ __forceinline__ void "
+       "Fun(TV V, TF F, TR RCT, TO Vdot) {\n";
+  std::set<int> used(md.a_used.begin(), md.a_used.end());
+  for (size_t q = 0; q < md.a_used.size(); q++)
+    o << "  const double A" << md.a_used[q] << " = " << md.a_expr[q] << ";\n";
+  for (int i = 0; i < S.nvar; i++) {
+    std::string e = md.vdot_expr[i];
+    // A[j] -> Aj (locals)
+    std::string r;
+    for (size_t p = 0; p < e.size(); p++) {
+      if (e[p] == 'A' && p + 1 < e.size() && e[p + 1] == '[' &&
+          (p == 0 || !(isalnum((unsigned char)e[p - 1]) || e[p - 1] == '_'))) {
+        size_t q = e.find(']', p);
+        r += "A" + e.substr(p + 2, q - p - 2);
+        p = q;
+      } else {
+        r += e[p];
+      }
+    }
+    o << "  Vdot[" << i << "] = " << r << ";\n";
+  }
+  o << "}\n";
+  // Jac_SP fused with the Ghimj build of ros_PrepareMatrix
+  // (int/rosenbrock.c:686-704): G = -Jac, G[diag] += ghinv.  Returns the
+  // first row (1-based) whose diagonal is exactly zero, the KppDecomp
+  // singularity test of util/sutil.c:8, which reads the diagonal before that
+  // row is eliminated.
+  auto emit_B = [&](std::ostream& oo) {
+    for (size_t q = 0; q < md.b_expr.size(); q++)
+      oo << "  const double B" << q << " = " << md.b_expr[q] << ";\n";
+  };
+  auto bsub = [&](const std::string& e) {
+    std::string r;
+    for (size_t p = 0; p < e.size(); p++) {
+      if (e[p] == 'B' && p + 1 < e.size() && e[p + 1] == '[' &&
+          (p == 0 || !(isalnum((unsigned char)e[p - 1]) || e[p - 1] == '_'))) {
+        size_t q = e.find(']', p);
+        r += "B" + e.substr(p + 2, q - p - 2);
+        p = q;
+      } else {
+        r += e[p];
+      }
+    }
+    return r;
+  };
+  o << "template <class TV, class TF, class TR, class TJ>\nstatic __device__ __forceinline__ void "
+       "Jac_SP(TV V, TF F, TR RCT, TJ JVS) {\n";
+  emit_B(o);
+  for (int n = 0; n < S.lu_nonzero; n++) o << "  JVS[" << n << "] = " << bsub(md.jvs_expr[n]) << ";\n";
+  o << "}\n";
+  o << "template <class TV, class TF, class TR, class TG>\nstatic __device__ __forceinline__ int "
+       "JacGhimj(TV V, TF F, TR RCT, double ghinv, TG G) {\n";
+  emit_B(o);
+  std::set<int> diag(S.lu_diag.begin(), S.lu_diag.begin() + S.nvar);
+  std::map<int, int> diag_row;
+  for (int i = 0; i < S.nvar; i++) diag_row[S.lu_diag[i]] = i;
+  for (int n = 0; n < S.lu_nonzero; n++) {
+    std::string e = bsub(md.jvs_expr[n]);
+    if (diag.count(n)) {
+      o << "  const double g" << diag_row[n] << " = -(" << e << ")+ghinv;\n  G[" << n << "] = g"
+        << diag_row[n] << ";\n";
+    } else {
+      o << "  G[" << n << "] = -(" << e << ");\n";
+    }
+  }
+  o << "  int ising = 0;\n";
+  for (int i = S.nvar - 1; i >= 0; i--) o << "  ising = (g" << i << " == 0.0) ? " << i + 1 << " : ising;\n";
+  o << "  return ising;\n}\n";
+  // Ghimj rebuild from a stored Jacobian (used when Jac is kept)
+  // KppDecomp unrolled: row-wise IKJ elimination of util/sutil.c:2-23 with
+  // the work row W kept in place (fill closure guarantees every updated
+  // column is in the row's pattern).  l = W[j]/U_jj, W[c] -= l*U_jc is
+  // bitwise the reference's a=-W[j]/U_jj; W[j]=-a; W[c]+=a*U_jc.
+  for (int checked = 0; checked < 2; checked++) {
+    if (checked)
+      o << "// KppDecomp with the singularity test of util/sutil.c:8 at the start of\n"
+           "// each row; returns k+1 for the first row whose diagonal is exactly zero.\n"
+           "template <class TG>\nstatic __device__ __forceinline__ int KppDecompChecked(TG G) {\n";
+    else
+      o << "template <class TG>\nstatic __device__ __forceinline__ void KppDecomp(TG G) {\n";
+    for (int k = 0; k < S.nvar; k++) {
+      if (checked) o << "  if (G[" << S.lu_diag[k] << "] == 0.0) return " << k + 1 << ";\n";
+      std::map<int, int> pos;  // column -> position in row k
+      for (int kk = S.lu_crow[k]; kk < S.lu_crow[k + 1]; kk++) pos[S.lu_icol[kk]] = kk;
+      for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
+        int j = S.lu_icol[kk];
+        o << "  { const double l = G[" << kk << "]/G[" << S.lu_diag[j] << "]; G[" << kk << "] = l;";
+        for (int jj = S.lu_diag[j] + 1; jj < S.lu_crow[j + 1]; jj++) {
+          int c = S.lu_icol[jj];
+          auto it = pos.find(c);
+          if (it == pos.end()) die("LU pattern not closed under elimination");
+          o << " G[" << it->second << "] = G[" << it->second << "] - l*G[" << jj << "];";
+        }
+        o << " }\n";
+      }
+    }
+    if (checked) o << "  return 0;\n";
+    o << "}\n";
+  }
+  o << "template <class TG, class TX>\nstatic __device__ __forceinline__ void KppSolve(TG JVS, "
+       "TX X) {\n";
+  for (int i = 0; i < S.nvar; i++) {
+    int b = S.lu_crow[i], e = S.lu_diag[i];
+    if (b < e) {
+      o << "  X[" << i << "] = X[" << i << "]";
+      for (int j = b; j < e; j++) o << "-JVS[" << j << "]*X[" << S.lu_icol[j] << "]";
+      o << ";\n";
+    }
+  }
+  for (int i = S.nvar - 1; i >= 0; i--) {
+    int b = S.lu_diag[i] + 1, e = S.lu_crow[i + 1];
+    if (b < e) {
+      o << "  X[" << i << "] = (X[" << i << "]";
+      for (int j = b; j < e; j++) o << "-JVS[" << j << "]*X[" << S.lu_icol[j] << "]";
+      o << ")/JVS[" << S.lu_diag[i] << "];\n";
+    } else {
+      o << "  X[" << i << "] = X[" << i << "]/JVS[" << S.lu_diag[i] << "];\n";
+    }
+  }
+  o << "}\n";
+  o << "#undef ARR_abc\n#undef ARR_ab\n#undef ARR_ac\n#undef EP2\n#undef EP3\n#undef FALL\n";
+  o << "};  // struct " << ns << "\n";
+  o << "namespace " << ns << "_data {\n";
+  emit_int_array(o, "static const", "LU_ICOL", S.lu_icol);
+  emit_int_array(o, "static const", "LU_CROW", S.lu_crow);
+  emit_int_array(o, "static const", "LU_DIAG", S.lu_diag);
+  o << "static const char* const SPC_NAMES[" << S.nspc << "] = {";
+  for (int k = 0; k < S.nspc; k++) o << (k % 8 ? " " : "\n  ") << "\"" << m.spc(m.code[k]).name << "\",";
+  o << "};\n}\n";
+}
+
+void emit_stats(Model& md, const std::string& path) {
+  auto& S = md.S;
+  std::ofstream o(path);
+  long nused = (long)md.a_used.size();
+  o << "{\n  \"root\": \"" << md.m->root << "\",\n  \"NVAR\": " << S.nvar << ",\n  \"NFIX\": "
+    << S.nfix << ",\n  \"NSPEC\": " << S.nspc << ",\n  \"NREACT\": " << S.nreact
+    << ",\n  \"NONZERO\": " << S.nonzero << ",\n  \"LU_NONZERO\": " << S.lu_nonzero
+    << ",\n  \"NRATES_USED\": " << nused << ",\n  \"NB\": " << md.b_expr.size()
+    << ",\n  \"flops\": {\"fun\": " << md.fun_ops.total() << ", \"jac\": " << md.jac_ops.total()
+    << ", \"decomp\": " << md.lu_ops.total() << ", \"solve\": " << md.sol_ops.total()
+    << ", \"decomp_div\": " << md.lu_ops.div << "},\n  \"species\": [";
+  for (int k = 0; k < S.nspc; k++)
+    o << (k ? ", " : "") << "\"" << md.m->spc(md.m->code[k]).name << "\"";
+  o << "],\n  \"init\": [";
+  auto iv = init_texts(md);
+  for (int k = 0; k < S.nspc; k++) o << (k ? ", " : "") << "\"" << iv[k] << "\"";
+  o << "],\n  \"cfactor\": " << efmt(md.m->cfactor);
+  o << ",\n  \"lu_icol\": [";
+  for (size_t i = 0; i < S.lu_icol.size(); i++) o << (i ? "," : "") << S.lu_icol[i];
+  o << "],\n  \"lu_crow\": [";
+  for (size_t i = 0; i < S.lu_crow.size(); i++) o << (i ? "," : "") << S.lu_crow[i];
+  o << "],\n  \"lu_diag\": [";
+  for (size_t i = 0; i < S.lu_diag.size(); i++) o << (i ? "," : "") << S.lu_diag[i];
+  o << "],\n  \"rate_types\": [";
+  for (int j = 0; j < S.nreact; j++) o << (j ? "," : "") << md.m->rates[j].type;
+  o << "]\n}\n";
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  std::string in, out_dir = ".", root, kpp_home;
+  bool do_reorder = true;
+  for (int i = 1; i < argc; i++) {
+    std::string a = argv[i];
+    auto next = [&]() -> std::string {
+      if (i + 1 >= argc) die("missing value for " + a);
+      return argv[++i];
+    };
+    if (a == "-o") out_dir = next();
+    else if (a == "--root") root = next();
+    else if (a == "--kpp-home") kpp_home = next();
+    else if (a == "--no-reorder") do_reorder = false;
+    else if (a == "-h" || a == "--help") {
+      printf("usage: kppgen [--kpp-home DIR] [--root NAME] [-o OUTDIR] mechanism.{def,kpp}\n");
+      return 0;
+    } else in = a;
+  }
+  if (in.empty()) die("no mechanism file given");
+  if (root.empty()) {
+    auto b = in.find_last_of('/');
+    root = in.substr(b == std::string::npos ? 0 : b + 1);
+    auto d = root.find_last_of('.');
+    if (d != std::string::npos) root = root.substr(0, d);
+  }
+  if (kpp_home.empty() && getenv("KPP_HOME")) kpp_home = getenv("KPP_HOME");
+  Scan sc;
+  if (!kpp_home.empty()) sc.search.push_back(kpp_home + "/models");
+  sc.file(in);
+
+  Mech m;
+  m.root = root;
+  m.table.push_back({"???", DUMMY_SPC, ""});
+  for (auto& s : sc.sections) {
+    const std::string& c = s.cmd;
+    if (c == "DEFVAR" || c == "DEFRAD") declare_species(m, s.body, VAR_SPC);
+    else if (c == "DEFFIX") declare_species(m, s.body, FIX_SPC);
+    else if (c == "SETVAR" || c == "SETRAD") set_type(m, s.body, VAR_SPC);
+    else if (c == "SETFIX") set_type(m, s.body, FIX_SPC);
+    else if (c == "INITVALUES") init_values(m, s.body);
+    else if (c == "EQUATIONS")
+      for (auto& e : split_semis(s.body)) parse_equation(m, e);
+  }
+  if (m.rates.empty()) die("no equations");
+  Model md;
+  md.m = &m;
+  md.S = reorder(m, do_reorder);
+  md.build();
+  std::string base = out_dir + "/" + root;
+  emit_c_model(md, base + "_model.h");
+  emit_hip(md, base + "_hip.h");
+  emit_stats(md, base + "_stats.json");
+  printf("kppgen: %s NVAR=%d NFIX=%d NREACT=%d NONZERO=%d LU_NONZERO=%d\n", root.c_str(),
+         md.S.nvar, md.S.nfix, md.S.nreact, md.S.nonzero, md.S.lu_nonzero);
+  return 0;
+}

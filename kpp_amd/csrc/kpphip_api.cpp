@@ -1,0 +1,540 @@
+// C ABI of the batched Rosenbrock solver (declarations in include/kpphip.h).
+#include "kpphip.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "mech_ops.h"
+
+#define KPPHIP_MECH(N) extern "C" const kpphip::MechOps* kpphip_mech_##N();
+#include "kpphip_registry.inc"
+#undef KPPHIP_MECH
+
+namespace {
+
+using kpphip::MechOps;
+using kpphip::RosCoef;
+using kpphip::RosParams;
+
+const MechOps* (*const g_mechs[])() = {
+#define KPPHIP_MECH(N) kpphip_mech_##N,
+#include "kpphip_registry.inc"
+#undef KPPHIP_MECH
+};
+constexpr int g_nmechs = (int)(sizeof(g_mechs) / sizeof(g_mechs[0]));
+
+const MechOps* find_mech(const char* name) {
+  if (!name) return nullptr;
+  for (int i = 0; i < g_nmechs; i++) {
+    const MechOps* m = g_mechs[i]();
+    if (strcmp(m->name, name) == 0) return m;
+  }
+  return nullptr;
+}
+
+// Unit roundoff by the halving loop of util/blas.c:150-180 (WLAMCH('E')).
+double unit_roundoff() {
+  volatile double eps = pow(0.5, 16), s;
+  int i;
+  for (i = 17; i <= 80; i++) {
+    eps = eps * 0.5;
+    s = 1.0 + eps;
+    if (s <= 1.0) break;
+  }
+  if (i == 80) return 2.2e-16;
+  return eps * 2.0;
+}
+
+// Method tables of int/rosenbrock.c:779-1222 (numbers as printed there).
+bool fill_method(int code, RosCoef& m) {
+  memset(&m, 0, sizeof m);
+  switch (code) {
+    case 1: {  // Ros2 :779-830
+      const double g = 1.70710678118655;
+      m.S = 2;
+      m.A[0] = 1.0 / g;
+      m.C[0] = (-2.0) / g;
+      m.NewF[0] = 1; m.NewF[1] = 1;
+      m.M[0] = (3.0) / (2.0 * g); m.M[1] = (1.0) / (2.0 * g);
+      m.E[0] = 1.0 / (2.0 * g); m.E[1] = 1.0 / (2.0 * g);
+      m.ELO = 2.0;
+      m.Alpha[0] = 0.0; m.Alpha[1] = 1.0;
+      m.Gamma[0] = g; m.Gamma[1] = -g;
+      return true;
+    }
+    case 2:  // Ros3 :834-892
+      m.S = 3;
+      m.A[0] = 1.0; m.A[1] = 1.0; m.A[2] = 0.0;
+      m.C[0] = -1.0156171083877702091975600115545;
+      m.C[1] = 4.0759956452537699824805835358067;
+      m.C[2] = 9.2076794298330791242156818474003;
+      m.NewF[0] = 1; m.NewF[1] = 1; m.NewF[2] = 0;
+      m.M[0] = 1.0; m.M[1] = 6.1697947043828245592553615689730;
+      m.M[2] = -0.4277225654321857332623837380651;
+      m.E[0] = 0.5; m.E[1] = -2.9079558716805469821718236208017;
+      m.E[2] = 0.2235406989781156962736090927619;
+      m.ELO = 3.0;
+      m.Alpha[0] = 0.0; m.Alpha[1] = 0.43586652150845899941601945119356;
+      m.Alpha[2] = 0.43586652150845899941601945119356;
+      m.Gamma[0] = 0.43586652150845899941601945119356;
+      m.Gamma[1] = 0.24291996454816804366592249683314;
+      m.Gamma[2] = 2.1851380027664058511513169485832;
+      return true;
+    case 3:  // Ros4 :898-973
+      m.S = 4;
+      m.A[0] = 0.2000000000000000e+01; m.A[1] = 0.1867943637803922e+01;
+      m.A[2] = 0.2344449711399156; m.A[3] = m.A[1]; m.A[4] = m.A[2]; m.A[5] = 0.0;
+      m.C[0] = -0.7137615036412310e+01; m.C[1] = 0.2580708087951457e+01;
+      m.C[2] = 0.6515950076447975; m.C[3] = -0.2137148994382534e+01;
+      m.C[4] = -0.3214669691237626; m.C[5] = -0.6949742501781779;
+      m.NewF[0] = 1; m.NewF[1] = 1; m.NewF[2] = 1; m.NewF[3] = 0;
+      m.M[0] = 0.2255570073418735e+01; m.M[1] = 0.2870493262186792;
+      m.M[2] = 0.4353179431840180; m.M[3] = 0.1093502252409163e+01;
+      m.E[0] = -0.2815431932141155; m.E[1] = -0.7276199124938920e-01;
+      m.E[2] = -0.1082196201495311; m.E[3] = -0.1093502252409163e+01;
+      m.ELO = 4.0;
+      m.Alpha[0] = 0.0; m.Alpha[1] = 0.1145640000000000e+01;
+      m.Alpha[2] = 0.6552168638155900; m.Alpha[3] = m.Alpha[2];
+      m.Gamma[0] = 0.5728200000000000; m.Gamma[1] = -0.1769193891319233e+01;
+      m.Gamma[2] = 0.7592633437920482; m.Gamma[3] = -0.1049021087100450;
+      return true;
+    case 4:  // Rodas3 :976-1045
+      m.S = 4;
+      m.A[0] = 0.0; m.A[1] = 2.0; m.A[2] = 0.0; m.A[3] = 2.0; m.A[4] = 0.0; m.A[5] = 1.0;
+      m.C[0] = 4.0; m.C[1] = 1.0; m.C[2] = -1.0; m.C[3] = 1.0; m.C[4] = -1.0;
+      m.C[5] = -2.66666666666667;
+      m.NewF[0] = 1; m.NewF[1] = 0; m.NewF[2] = 1; m.NewF[3] = 1;
+      m.M[0] = 2.0; m.M[1] = 0.0; m.M[2] = 1.0; m.M[3] = 1.0;
+      m.E[3] = 1.0;
+      m.ELO = 3.0;
+      m.Alpha[2] = 1.0; m.Alpha[3] = 1.0;
+      m.Gamma[0] = 0.5; m.Gamma[1] = 1.5;
+      return true;
+    case 5:  // Rodas4 :1121-1222
+      m.S = 6;
+      m.Alpha[0] = 0.000; m.Alpha[1] = 0.386; m.Alpha[2] = 0.210;
+      m.Alpha[3] = 0.630; m.Alpha[4] = 1.000; m.Alpha[5] = 1.000;
+      m.Gamma[0] = 0.2500000000000000; m.Gamma[1] = -0.1043000000000000;
+      m.Gamma[2] = 0.1035000000000000; m.Gamma[3] = -0.3620000000000023e-01;
+      m.A[0] = 0.1544000000000000e+01; m.A[1] = 0.9466785280815826;
+      m.A[2] = 0.2557011698983284; m.A[3] = 0.3314825187068521e+01;
+      m.A[4] = 0.2896124015972201e+01; m.A[5] = 0.9986419139977817;
+      m.A[6] = 0.1221224509226641e+01; m.A[7] = 0.6019134481288629e+01;
+      m.A[8] = 0.1253708332932087e+02; m.A[9] = -0.6878860361058950;
+      m.A[10] = m.A[6]; m.A[11] = m.A[7]; m.A[12] = m.A[8]; m.A[13] = m.A[9]; m.A[14] = 1.0;
+      m.C[0] = -0.5668800000000000e+01; m.C[1] = -0.2430093356833875e+01;
+      m.C[2] = -0.2063599157091915; m.C[3] = -0.1073529058151375;
+      m.C[4] = -0.9594562251023355e+01; m.C[5] = -0.2047028614809616e+02;
+      m.C[6] = 0.7496443313967647e+01; m.C[7] = -0.1024680431464352e+02;
+      m.C[8] = -0.3399990352819905e+02; m.C[9] = 0.1170890893206160e+02;
+      m.C[10] = 0.8083246795921522e+01; m.C[11] = -0.7981132988064893e+01;
+      m.C[12] = -0.3152159432874371e+02; m.C[13] = 0.1631930543123136e+02;
+      m.C[14] = -0.6058818238834054e+01;
+      m.M[0] = m.A[6]; m.M[1] = m.A[7]; m.M[2] = m.A[8]; m.M[3] = m.A[9];
+      m.M[4] = 1.0; m.M[5] = 1.0;
+      m.E[5] = 1.0;
+      for (int i = 0; i < 6; i++) m.NewF[i] = 1;
+      m.ELO = 4.0;
+      return true;
+  }
+  return false;
+}
+
+int device_cus() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  return n;
+}
+
+constexpr size_t kHead = 256;  // queue counter
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Geometry {
+  int nblocks;
+  int refill_min;
+  size_t tol_bytes, tile_bytes, total;
+};
+
+bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry& g) {
+  const int rows = m->rows(S);
+  if (rows <= 0) return false;
+  const int variant = l ? l->variant : KPP_VARIANT_PLAIN;
+  if (variant == KPP_VARIANT_PERSISTENT) {
+    int nb = (l && l->nblocks > 0) ? l->nblocks : 0;
+    if (nb == 0) {
+      const int cus = device_cus();
+      nb = (cus > 0 ? cus : 256) * 4;
+    }
+    const int need = (ncells + 63) / 64;
+    if (nb > need) nb = need > 0 ? need : 1;
+    g.nblocks = nb;
+    g.refill_min = (l && l->refill_min > 0) ? l->refill_min : 16;
+  } else if (variant == KPP_VARIANT_PLAIN) {
+    g.nblocks = (ncells + 63) / 64;
+    g.refill_min = 0;
+  } else {
+    return false;
+  }
+  g.tol_bytes = align256(2 * (size_t)m->nvar * sizeof(double));
+  g.tile_bytes = (size_t)rows * 64 * sizeof(double);
+  g.total = kHead + g.tol_bytes + (size_t)g.nblocks * g.tile_bytes;
+  return true;
+}
+
+int check_params(const int* IPAR, const double* RPAR, const double* atol, const double* rtol,
+                 int nvar) {
+  // order and codes of int/rosenbrock.c:286-382
+  const int max_steps = (IPAR[2] == 0) ? 100000 : IPAR[2];
+  if (max_steps < 0) return -1;
+  if ((IPAR[3] < 1) || (IPAR[3] > 5)) return -2;
+  const double roundoff = unit_roundoff();
+  if (RPAR[0] < 0.0) return -3;
+  if (RPAR[1] < 0.0) return -3;
+  if (RPAR[2] < 0.0) return -3;
+  for (int i = 3; i <= 6; i++)
+    if (RPAR[i] < 0.0) return -4;
+  const int ntol = (IPAR[1] == 0) ? nvar : 1;
+  for (int i = 0; i < ntol; i++)
+    if ((atol[i] <= 0.0) || (rtol[i] <= 10.0 * roundoff) || (rtol[i] >= 1.0)) return -5;
+  return 0;
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kpp_mech_count(void) { return g_nmechs; }
+const char* kpp_mech_name(int i) { return (i >= 0 && i < g_nmechs) ? g_mechs[i]()->name : nullptr; }
+
+int kpp_mech_info(const char* mech, int out[8]) {
+  const MechOps* m = find_mech(mech);
+  if (!m) return KPP_E_MECH;
+  if (!out) return KPP_E_ARG;
+  out[0] = m->nvar; out[1] = m->nfix; out[2] = m->nspec; out[3] = m->nreact;
+  out[4] = m->nonzero; out[5] = m->lu_nonzero; out[6] = m->rates_use_time;
+  out[7] = m->n_sun_rates;
+  return 0;
+}
+
+const char* kpp_mech_species(const char* mech, int k) {
+  const MechOps* m = find_mech(mech);
+  if (!m || k < 0 || k >= m->nspec) return nullptr;
+  return m->species[k];
+}
+
+int kpp_mech_lu_pattern(const char* mech, int* icol, int* crow, int* diag) {
+  const MechOps* m = find_mech(mech);
+  if (!m) return KPP_E_MECH;
+  if (icol) memcpy(icol, m->lu_icol, sizeof(int) * m->lu_nonzero);
+  if (crow) memcpy(crow, m->lu_crow, sizeof(int) * (m->nvar + 1));
+  if (diag) memcpy(diag, m->lu_diag, sizeof(int) * (m->nvar + 1));
+  return 0;
+}
+
+int kpp_check_params(const int IPAR[20], const double RPAR[20], const double* atol,
+                     const double* rtol, int nvar) {
+  if (!IPAR || !RPAR || !atol || !rtol || nvar <= 0) return KPP_E_ARG;
+  return check_params(IPAR, RPAR, atol, rtol, nvar);
+}
+
+size_t kpp_workspace_bytes(const char* mech, int method, const kpp_launch* l, int ncells) {
+  const MechOps* m = find_mech(mech);
+  RosCoef c;
+  Geometry g;
+  if (!m || !fill_method(method, c) || ncells < 0) return 0;
+  if (!geometry(m, c.S, l, ncells, g)) return 0;
+  return g.total;
+}
+
+int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
+                       const double RPAR[20], const double* atol, const double* rtol,
+                       const kpp_launch* l, void* workspace, size_t workspace_bytes,
+                       void* stream) {
+  const MechOps* m = find_mech(mech);
+  if (!m) return KPP_E_MECH;
+  if (!b || !IPAR || !RPAR || !atol || !rtol || b->ncells < 0 || b->ld < b->ncells)
+    return KPP_E_ARG;
+  if (b->ncells > 0 && (!b->C || !b->temp || !b->cfactor || !b->ierr)) return KPP_E_ARG;
+  if (b->sun_mode == KPP_SUN_FIXED && b->ncells > 0 && !b->sun && !b->rconst) return KPP_E_ARG;
+  if (b->sun_mode != KPP_SUN_FIXED && b->sun_mode != KPP_SUN_TOD) return KPP_E_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int e = check_params(IPAR, RPAR, atol, rtol, m->nvar);
+  if (e != 0) {
+    // Rosenbrock returns before integrating; every cell reports the code
+    if (b->ncells > 0)
+      hipLaunchKernelGGL(k_fill_int, dim3((b->ncells + 255) / 256), dim3(256), 0, st, b->ierr,
+                         b->ncells, e);
+    return e;
+  }
+  if (b->ncells == 0) return 0;
+  RosParams p;
+  memset(&p, 0, sizeof p);
+  if (!fill_method(IPAR[3], p.coef)) return -2;
+  Geometry g;
+  if (!geometry(m, p.coef.S, l, b->ncells, g)) return KPP_E_ARG;
+  if (!workspace || workspace_bytes < g.total) return KPP_E_WORKSPACE;
+  char* ws = (char*)workspace;
+  const int ntol = (IPAR[1] == 0) ? m->nvar : 1;
+  double* dtol = (double*)(ws + kHead);
+  if (hipMemcpyAsync(dtol, atol, sizeof(double) * ntol, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(dtol + m->nvar, rtol, sizeof(double) * ntol, hipMemcpyHostToDevice, st) !=
+          hipSuccess)
+    return KPP_E_HIP;
+  p.ncells = b->ncells;
+  p.ldc = b->ld;
+  p.C = b->C;
+  p.temp = b->temp;
+  p.cfactor = b->cfactor;
+  p.sun = b->sun;
+  p.sun_mode = b->sun_mode;
+  p.tstart = b->tstart;
+  p.tend = b->tend;
+  p.tstart0 = b->tstart0;
+  p.tend0 = b->tend0;
+  p.atol = dtol;
+  p.rtol = dtol + m->nvar;
+  p.vector_tol = (IPAR[1] == 0);
+  p.autonomous = !(IPAR[0] == 0);
+  p.max_steps = (IPAR[2] == 0) ? 100000 : IPAR[2];
+  p.Roundoff = unit_roundoff();
+  p.Hmin = RPAR[0];
+  p.Hmax_user = RPAR[1];
+  p.Hstart_user = RPAR[2];
+  p.FacMin = (RPAR[3] == 0.0) ? 0.2 : RPAR[3];
+  p.FacMax = (RPAR[4] == 0.0) ? 6.0 : RPAR[4];
+  p.FacRej = (RPAR[5] == 0.0) ? 0.1 : RPAR[5];
+  p.FacSafe = (RPAR[6] == 0.0) ? 0.9 : RPAR[6];
+  p.hstart = b->hstart;
+  p.rconst = b->rconst;
+  p.rates_given = b->rconst != nullptr;
+  // dF/dT is exactly zero when no rate constant depends on T
+  p.dfdt_zero = p.rates_given || (b->sun_mode == KPP_SUN_FIXED && !m->rates_use_time);
+  for (int i = 0; i < 8; i++) p.stat0[i] = IPAR[10 + i];
+  p.ierr = b->ierr;
+  p.istat = b->istat;
+  p.rstat = b->rstat;
+  p.order = b->order;
+  p.queue = (int*)ws;
+  p.scratch = (double*)(ws + kHead + g.tol_bytes);
+  p.refill_min = g.refill_min;
+  const int variant = l ? l->variant : KPP_VARIANT_PLAIN;
+  if (m->launch(p, variant, g.nblocks, st) != hipSuccess) return KPP_E_HIP;
+  return 0;
+}
+
+static int component(const char* mech, int which, int n, int ld, const double* a, const double* b,
+                     const double* c, const double* d, double* out, int* iout, void* stream) {
+  const MechOps* m = find_mech(mech);
+  if (!m) return KPP_E_MECH;
+  if (n < 0 || ld < n) return KPP_E_ARG;
+  if (m->component(which, n, (size_t)ld, a, b, c, d, out, iout, (hipStream_t)stream) != hipSuccess)
+    return KPP_E_HIP;
+  return 0;
+}
+
+int kpp_fun_dev(const char* mech, int n, int ld, const double* V, const double* F,
+                const double* RCT, double* Vdot, void* stream) {
+  return component(mech, KPP_COMP_FUN, n, ld, V, F, RCT, nullptr, Vdot, nullptr, stream);
+}
+int kpp_jac_sp_dev(const char* mech, int n, int ld, const double* V, const double* F,
+                   const double* RCT, double* JVS, void* stream) {
+  return component(mech, KPP_COMP_JAC, n, ld, V, F, RCT, nullptr, JVS, nullptr, stream);
+}
+int kpp_decomp_dev(const char* mech, int n, int ld, double* JVS, int* ising, void* stream) {
+  return component(mech, KPP_COMP_DECOMP, n, ld, nullptr, nullptr, nullptr, nullptr, JVS, ising,
+                   stream);
+}
+int kpp_solve_dev(const char* mech, int n, int ld, const double* JVS, double* X, void* stream) {
+  return component(mech, KPP_COMP_SOLVE, n, ld, JVS, nullptr, nullptr, nullptr, X, nullptr,
+                   stream);
+}
+int kpp_rates_dev(const char* mech, int n, int ld, const double* temp, const double* cfactor,
+                  const double* sun, const double* time, double* RCONST, void* stream) {
+  return component(mech, KPP_COMP_RATES, n, ld, temp, cfactor, sun, time, RCONST, nullptr,
+                   stream);
+}
+
+// ---------------------------------------------------------------------------
+// Host-buffer context API
+// ---------------------------------------------------------------------------
+struct kpp_ctx {
+  const MechOps* m;
+  int ncells, method, sun_mode;
+  double *C, *temp, *cfactor, *sun, *tstart, *tend, *hstart, *rstat;
+  int *ierr, *istat;
+  void* ws;
+  size_t ws_bytes;
+  std::vector<double> atol, rtol;
+  int vector_tol;
+};
+
+static void ctx_free(kpp_ctx* c) {
+  if (!c) return;
+  void* ptrs[] = {c->C, c->temp, c->cfactor, c->sun, c->tstart, c->tend,
+                  c->hstart, c->rstat, c->ierr, c->istat, c->ws};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete c;
+}
+
+int kpp_ctx_create(kpp_ctx** out, const char* mech, int ncells, int method, int flags) {
+  if (!out || ncells <= 0) return KPP_E_ARG;
+  *out = nullptr;
+  const MechOps* m = find_mech(mech);
+  if (!m) return KPP_E_MECH;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return KPP_E_NODEV;
+  kpp_ctx* c = new kpp_ctx();  // value-initialised: all pointers null
+  c->m = m;
+  c->ncells = ncells;
+  c->method = method;
+  c->sun_mode = (flags & KPP_SUN_TOD) ? KPP_SUN_TOD : KPP_SUN_FIXED;
+  c->atol.assign(1, 1.0);
+  c->rtol.assign(1, 1.0e-3);
+  c->vector_tol = 0;
+  const size_t n = (size_t)ncells;
+  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0};
+  c->ws_bytes = kpp_workspace_bytes(mech, method, &l, ncells);
+  if (c->ws_bytes == 0) { ctx_free(c); return -2; }
+  bool ok = hipMalloc(&c->C, sizeof(double) * n * m->nspec) == hipSuccess &&
+            hipMalloc(&c->temp, sizeof(double) * n) == hipSuccess &&
+            hipMalloc(&c->cfactor, sizeof(double) * n) == hipSuccess &&
+            hipMalloc(&c->sun, sizeof(double) * n) == hipSuccess &&
+            hipMalloc(&c->tstart, sizeof(double) * n) == hipSuccess &&
+            hipMalloc(&c->tend, sizeof(double) * n) == hipSuccess &&
+            hipMalloc(&c->hstart, sizeof(double) * n) == hipSuccess &&
+            hipMalloc(&c->rstat, sizeof(double) * 2 * n) == hipSuccess &&
+            hipMalloc(&c->ierr, sizeof(int) * n) == hipSuccess &&
+            hipMalloc(&c->istat, sizeof(int) * 8 * n) == hipSuccess &&
+            hipMalloc(&c->ws, c->ws_bytes) == hipSuccess;
+  if (!ok) { ctx_free(c); return KPP_E_HIP; }
+  if (hipMemset(c->hstart, 0, sizeof(double) * n) != hipSuccess ||
+      hipMemset(c->sun, 0, sizeof(double) * n) != hipSuccess) {
+    ctx_free(c);
+    return KPP_E_HIP;
+  }
+  *out = c;
+  return 0;
+}
+
+void kpp_ctx_destroy(kpp_ctx* c) { ctx_free(c); }
+
+int kpp_set_state(kpp_ctx* c, const double* C, const double* temp, const double* cfactor,
+                  const double* sun) {
+  if (!c || !C || !temp || !cfactor) return KPP_E_ARG;
+  if (c->sun_mode == KPP_SUN_FIXED && !sun) return KPP_E_ARG;
+  const size_t n = (size_t)c->ncells;
+  if (hipMemcpy(c->C, C, sizeof(double) * n * c->m->nspec, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->temp, temp, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->cfactor, cfactor, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+    return KPP_E_HIP;
+  if (sun && hipMemcpy(c->sun, sun, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+    return KPP_E_HIP;
+  return 0;
+}
+
+static int ctx_run(kpp_ctx* c, const int* IPAR, const double* RPAR, const double* atol,
+                   const double* rtol, bool use_hstart) {
+  kpp_batch b;
+  memset(&b, 0, sizeof b);
+  b.ncells = c->ncells;
+  b.ld = c->ncells;
+  b.C = c->C;
+  b.temp = c->temp;
+  b.cfactor = c->cfactor;
+  b.sun = c->sun;
+  b.sun_mode = c->sun_mode;
+  b.tstart = c->tstart;
+  b.tend = c->tend;
+  b.hstart = use_hstart ? c->hstart : nullptr;
+  b.ierr = c->ierr;
+  b.istat = c->istat;
+  b.rstat = c->rstat;
+  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0};
+  int r = kpp_rosenbrock_dev(c->m->name, &b, IPAR, RPAR, atol, rtol, &l, c->ws, c->ws_bytes,
+                             nullptr);
+  if (hipDeviceSynchronize() != hipSuccess) return KPP_E_HIP;
+  return r;
+}
+
+int kpp_rosenbrock_batch(kpp_ctx* c, const double* tstart, const double* tend,
+                         const double* atol, const double* rtol, const int IPAR[20],
+                         const double RPAR[20], const double* hstart) {
+  if (!c || !tstart || !tend || !atol || !rtol || !IPAR || !RPAR) return KPP_E_ARG;
+  if (IPAR[3] != c->method) return KPP_E_ARG;
+  const size_t n = (size_t)c->ncells;
+  if (hipMemcpy(c->tstart, tstart, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->tend, tend, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+    return KPP_E_HIP;
+  if (hstart &&
+      hipMemcpy(c->hstart, hstart, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+    return KPP_E_HIP;
+  return ctx_run(c, IPAR, RPAR, atol, rtol, hstart != nullptr);
+}
+
+int kpp_get_state(kpp_ctx* c, double* C, int* ierr, int* istat, double* rstat) {
+  if (!c) return KPP_E_ARG;
+  const size_t n = (size_t)c->ncells;
+  if (C && hipMemcpy(C, c->C, sizeof(double) * n * c->m->nspec, hipMemcpyDeviceToHost) != hipSuccess)
+    return KPP_E_HIP;
+  if (ierr && hipMemcpy(ierr, c->ierr, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    return KPP_E_HIP;
+  if (istat && hipMemcpy(istat, c->istat, sizeof(int) * 8 * n, hipMemcpyDeviceToHost) != hipSuccess)
+    return KPP_E_HIP;
+  if (rstat && hipMemcpy(rstat, c->rstat, sizeof(double) * 2 * n, hipMemcpyDeviceToHost) != hipSuccess)
+    return KPP_E_HIP;
+  return 0;
+}
+
+int kpp_ctx_set_tol(kpp_ctx* c, const double* atol, const double* rtol, int vector_tol) {
+  if (!c || !atol || !rtol) return KPP_E_ARG;
+  const int n = vector_tol ? c->m->nvar : 1;
+  c->atol.assign(atol, atol + n);
+  c->rtol.assign(rtol, rtol + n);
+  c->vector_tol = vector_tol ? 1 : 0;
+  return 0;
+}
+
+__global__ static void k_fill_time(double* ts, double* te, int n, double t0, double t1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) { ts[i] = t0; te[i] = t1; }
+}
+__global__ static void k_copy_hexit(double* hs, const double* rstat, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) hs[i] = rstat[n + i];
+}
+
+// INTEGRATE (int/rosenbrock.c:94-133): non-autonomous, RPAR[2] = STEPMIN
+// (per cell: the previous call's Hexit), method fixed at context creation
+// (the reference hard-codes 5 = Rodas4), tolerances of the context.
+int INTEGRATE_BATCH(kpp_ctx* c, double TIN, double TOUT) {
+  if (!c) return KPP_E_ARG;
+  int IPAR[20];
+  double RPAR[20];
+  memset(IPAR, 0, sizeof IPAR);
+  memset(RPAR, 0, sizeof RPAR);
+  IPAR[0] = 0;
+  IPAR[1] = c->vector_tol ? 0 : 1;
+  IPAR[3] = c->method;
+  const int nb = (c->ncells + 255) / 256;
+  hipLaunchKernelGGL(k_fill_time, dim3(nb), dim3(256), 0, 0, c->tstart, c->tend, c->ncells, TIN,
+                     TOUT);
+  int r = ctx_run(c, IPAR, RPAR, c->atol.data(), c->rtol.data(), true);
+  if (r != 0) return r;
+  hipLaunchKernelGGL(k_copy_hexit, dim3(nb), dim3(256), 0, 0, c->hstart, c->rstat, c->ncells);
+  if (hipDeviceSynchronize() != hipSuccess) return KPP_E_HIP;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+// Per-mechanism kernels and launchers.  Included by one generated
+// translation unit per mechanism after the mechanism's <root>_hip.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mech_ops.h"
+#include "ros_device.h"
+
+namespace kpphip {
+
+// Accessor over the caller's SoA arrays: element i of cell c at b[i*ld + c].
+struct Strided {
+  double* __restrict__ b;
+  size_t ld;
+  __device__ __forceinline__ double& operator[](int i) const { return b[(size_t)i * ld]; }
+};
+
+// Batched model functions with the generated-model ABI of KPP
+// (Fun/Jac_SP/KppDecomp/KppSolve/Update_RCONST; prototypes in
+// int/rosenbrock.c:54-91), one cell per lane over SoA arrays.
+template <class M>
+__global__ __launch_bounds__(256) void k_fun(int n, size_t ld, const double* V, const double* F,
+                                             const double* R, double* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Strided v{(double*)V + c, ld}, f{(double*)F + c, ld}, r{(double*)R + c, ld}, o{out + c, ld};
+  M::Fun(v, f, r, o);
+}
+template <class M>
+__global__ __launch_bounds__(256) void k_jac(int n, size_t ld, const double* V, const double* F,
+                                             const double* R, double* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Strided v{(double*)V + c, ld}, f{(double*)F + c, ld}, r{(double*)R + c, ld}, o{out + c, ld};
+  M::Jac_SP(v, f, r, o);
+}
+template <class M>
+__global__ __launch_bounds__(256) void k_decomp(int n, size_t ld, double* JVS, int* ising) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Strided g{JVS + c, ld};
+  ising[c] = M::KppDecompChecked(g);
+}
+template <class M>
+__global__ __launch_bounds__(256) void k_solve(int n, size_t ld, const double* JVS, double* X) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Strided g{(double*)JVS + c, ld}, x{X + c, ld};
+  M::KppSolve(g, x);
+}
+// Update_RCONST at (TEMP, CFACTOR, SUN, TIME) per cell.  NUMBER-type rates
+// are constants inside Fun/Jac_SP and their slot is left as zero.
+template <class M>
+__global__ __launch_bounds__(256) void k_rates(int n, size_t ld, const double* temp,
+                                               const double* cfactor, const double* sun,
+                                               const double* time, double* R) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  Strided r{R + c, ld};
+  for (int i = 0; i < M::NREACT; i++) r[i] = 0.0;
+  M::rates_fixed(temp[c], cfactor[c], r);
+  M::rates_sun(temp[c], cfactor[c], sun[c], time ? time[c] : 0.0, r);
+}
+
+static __global__ void k_zero_int(int* q) { *q = 0; }
+
+template <class M, int S>
+hipError_t launch_s(const RosParams& p, int variant, int nblocks, hipStream_t st) {
+  if (variant == KPP_VARIANT_PERSISTENT) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, p.queue);
+    hipLaunchKernelGGL((ros_persistent_kernel<M, S>), dim3(nblocks), dim3(TILE), 0, st, p);
+  } else {
+    hipLaunchKernelGGL((ros_plain_kernel<M, S>), dim3(nblocks), dim3(TILE), 0, st, p);
+  }
+  return hipGetLastError();
+}
+
+template <class M>
+hipError_t launch_mech(const RosParams& p, int variant, int nblocks, hipStream_t st) {
+  switch (p.coef.S) {
+    case 2: return launch_s<M, 2>(p, variant, nblocks, st);
+    case 3: return launch_s<M, 3>(p, variant, nblocks, st);
+    case 4: return launch_s<M, 4>(p, variant, nblocks, st);
+    case 6: return launch_s<M, 6>(p, variant, nblocks, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class M>
+int rows_mech(int S) {
+  switch (S) {
+    case 2: return Layout<M, 2>::ROWS;
+    case 3: return Layout<M, 3>::ROWS;
+    case 4: return Layout<M, 4>::ROWS;
+    case 6: return Layout<M, 6>::ROWS;
+  }
+  return -1;
+}
+
+template <class M>
+hipError_t component_mech(int which, int n, size_t ld, const double* a, const double* b,
+                          const double* c, const double* d, double* out, int* iout,
+                          hipStream_t st) {
+  const int bs = 256, nb = (n + bs - 1) / bs;
+  if (n <= 0) return hipSuccess;
+  switch (which) {
+    case KPP_COMP_FUN: hipLaunchKernelGGL(k_fun<M>, dim3(nb), dim3(bs), 0, st, n, ld, a, b, c, out); break;
+    case KPP_COMP_JAC: hipLaunchKernelGGL(k_jac<M>, dim3(nb), dim3(bs), 0, st, n, ld, a, b, c, out); break;
+    case KPP_COMP_DECOMP: hipLaunchKernelGGL(k_decomp<M>, dim3(nb), dim3(bs), 0, st, n, ld, out, iout); break;
+    case KPP_COMP_SOLVE: hipLaunchKernelGGL(k_solve<M>, dim3(nb), dim3(bs), 0, st, n, ld, a, out); break;
+    case KPP_COMP_RATES:
+      hipLaunchKernelGGL(k_rates<M>, dim3(nb), dim3(bs), 0, st, n, ld, a, b, c, d, out);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <class M, class D>
+MechOps make_ops(const char* name) {
+  MechOps o;
+  o.name = name;
+  o.nvar = M::NVAR;
+  o.nfix = M::NFIX;
+  o.nspec = M::NSPEC;
+  o.nreact = M::NREACT;
+  o.nonzero = M::NONZERO;
+  o.lu_nonzero = M::LU_NONZERO;
+  o.rates_use_time = M::RATES_USE_TIME ? 1 : 0;
+  o.n_sun_rates = M::N_SUN_RATES;
+  o.species = D::names();
+  o.lu_icol = D::icol();
+  o.lu_crow = D::crow();
+  o.lu_diag = D::diag();
+  o.rows = rows_mech<M>;
+  o.launch = launch_mech<M>;
+  o.component = component_mech<M>;
+  return o;
+}
+
+}  // namespace kpphip
+
+// Defines the registry hook kpphip_mech_<NAME>() for mechanism NAME.
+#define KPPHIP_DEFINE_MECH(NAME)                                                        \
+  namespace {                                                                           \
+  struct Data_##NAME {                                                                  \
+    static const char* const* names() { return kppm_##NAME##_data::SPC_NAMES; }         \
+    static const int* icol() { return kppm_##NAME##_data::LU_ICOL; }                    \
+    static const int* crow() { return kppm_##NAME##_data::LU_CROW; }                    \
+    static const int* diag() { return kppm_##NAME##_data::LU_DIAG; }                    \
+  };                                                                                    \
+  }                                                                                     \
+  extern "C" const kpphip::MechOps* kpphip_mech_##NAME() {                              \
+    static const kpphip::MechOps ops = kpphip::make_ops<kppm_##NAME, Data_##NAME>(#NAME); \
+    return &ops;                                                                        \
+  }

@@ -1,0 +1,32 @@
+// Host-visible description of one compiled mechanism.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "kpphip_params.h"
+
+#define KPP_VARIANT_PLAIN 0
+#define KPP_VARIANT_PERSISTENT 1
+
+#define KPP_COMP_FUN 0
+#define KPP_COMP_JAC 1
+#define KPP_COMP_DECOMP 2
+#define KPP_COMP_SOLVE 3
+#define KPP_COMP_RATES 4
+
+namespace kpphip {
+
+struct MechOps {
+  const char* name;
+  int nvar, nfix, nspec, nreact, nonzero, lu_nonzero;
+  int rates_use_time, n_sun_rates;
+  const char* const* species;
+  const int *lu_icol, *lu_crow, *lu_diag;
+  int (*rows)(int S);  // tile rows per lane for S stages
+  hipError_t (*launch)(const RosParams& p, int variant, int nblocks, hipStream_t st);
+  hipError_t (*component)(int which, int n, size_t ld, const double* a, const double* b,
+                          const double* c, const double* d, double* out, int* iout,
+                          hipStream_t st);
+};
+
+}  // namespace kpphip

@@ -1,0 +1,390 @@
+// Batched Rosenbrock integrator for gfx950: device side.
+//
+// One lane integrates one grid cell with the algorithm of KPP's
+// int/rosenbrock.c (RosenbrockIntegrator :450-627, ros_PrepareMatrix
+// :686-737, ros_ErrorNorm :632-659, ros_FunTimeDerivative :663-682); the
+// model functions are the mechanism compiler's straight-line HIP output.
+//
+// Control flow.  The integrator is written as a state machine that performs
+// ONE step attempt per iteration of the lane's loop.  All lanes of a
+// wavefront execute the same attempt body whatever step their cell is at,
+// so a wavefront only diverges at accept/reject and when a lane's cell ends.
+// The plain kernel gives each lane one cell; the persistent kernel refills
+// finished lanes from a global queue (batched with a wavefront ballot), so
+// its wavefronts stay full while uneven cells drain.
+//
+// Memory layout.  Per-cell working vectors live in HBM as 64-lane tiles:
+// element i of lane l of tile t is at scratch[(t*ROWS + row + i)*64 + l], so
+// a wavefront-wide access of one element is one contiguous 512-byte segment
+// and element offsets are compile-time constants from the lane's base.
+//
+// Arithmetic follows the reference operation by operation (comments at each
+// step), so with FP contraction off and the same rate constants the results
+// are the CPU oracle's bit for bit; exp/pow/cos from the device math library
+// are the only source of difference.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kpphip_params.h"
+
+namespace kpphip {
+
+constexpr int TILE = 64;
+
+// Element accessor for a lane inside a 64-lane tile.
+struct Acc {
+  double* __restrict__ b;
+  __device__ __forceinline__ double& operator[](int i) const { return b[i * TILE]; }
+};
+
+// Row layout of a tile for mechanism M and S stages.
+template <class M, int S>
+struct Layout {
+  static constexpr int Y = 0;
+  static constexpr int FIX = Y + M::NVAR;
+  static constexpr int RCT = FIX + (M::NFIX > 0 ? M::NFIX : 1);
+  static constexpr int FCN0 = RCT + M::NREACT;
+  static constexpr int FCN = FCN0 + M::NVAR;
+  static constexpr int DFDT = FCN + M::NVAR;
+  static constexpr int YNEW = DFDT + M::NVAR;
+  static constexpr int K = YNEW + M::NVAR;
+  static constexpr int G = K + S * M::NVAR;
+  static constexpr int ROWS = G + M::LU_NONZERO;
+};
+
+// util/UpdateSun.c:1-20
+__device__ __forceinline__ double update_sun(double TIME) {
+  const double SunRise = 4.5, SunSet = 19.5;
+  const double PI = 3.14159265358979;
+  double Thour = TIME / 3600.0;
+  double Tlocal = Thour - (double)(((int)Thour / 24) * 24);
+  if ((Tlocal >= SunRise) && (Tlocal <= SunSet)) {
+    double Ttmp = (2.0 * Tlocal - SunRise - SunSet) / (SunSet - SunRise);
+    if (Ttmp > 0) Ttmp = Ttmp * Ttmp;
+    else Ttmp = -Ttmp * Ttmp;
+    return (1.0 + cos(PI * Ttmp)) / 2.0;
+  }
+  return 0.0;
+}
+
+// MAX/MIN/ABS macros of int/rosenbrock.c:1-3 (not fmin/fmax: NaN order)
+__device__ __forceinline__ double dmax(double a, double b) { return (a >= b) ? a : b; }
+__device__ __forceinline__ double dmin(double b, double c) { return (b < c) ? b : c; }
+__device__ __forceinline__ double dabs(double x) { return (x >= 0) ? x : -x; }
+
+// Model calls kept out of line: one copy of each large straight-line routine
+// per kernel bounds instruction-cache footprint.
+template <class M>
+__device__ __noinline__ __attribute__((used)) void call_fun(Acc V, Acc F, Acc R, Acc out) {
+  M::Fun(V, F, R, out);
+}
+template <class M>
+__device__ __noinline__ __attribute__((used)) int call_jac_ghimj(Acc V, Acc F, Acc R, double ghinv, Acc G) {
+  return M::JacGhimj(V, F, R, ghinv, G);
+}
+template <class M>
+__device__ __noinline__ __attribute__((used)) void call_decomp(Acc G) {
+  M::KppDecomp(G);
+}
+template <class M>
+__device__ __noinline__ __attribute__((used)) void call_solve(Acc G, Acc X) {
+  M::KppSolve(G, X);
+}
+template <class M>
+__device__ __noinline__ __attribute__((used)) void call_rates_sun(double TEMP, double CFACTOR, double SUN, double TIME,
+                                            Acc R) {
+  M::rates_sun(TEMP, CFACTOR, SUN, TIME, R);
+}
+template <class M>
+__device__ __noinline__ __attribute__((used)) void call_rates_fixed(double TEMP, double CFACTOR, Acc R) {
+  M::rates_fixed(TEMP, CFACTOR, R);
+}
+
+// Per-lane integration state between attempts.
+struct CellState {
+  int cell;
+  double T, Tstart, Tend, H, Hmax, Hexit;
+  double TEMP, CFACTOR, SUN;
+  int Direction;
+  bool RejectLastH, RejectMoreH, newstep;
+  int Nfun, Njac, Nstp, Nacc, Nrej, Ndec, Nsol, Nsng;
+};
+
+template <class M, int S>
+struct Cell {
+  using L = Layout<M, S>;
+  double* base;  // lane base pointer inside its tile
+  __device__ __forceinline__ Acc at(int row) const { return Acc{base + row * TILE}; }
+
+  // Rate constants at time T (int/rosenbrock.c:1257-1296 call Update_SUN and
+  // Update_RCONST before every Fun/Jac).  Rates free of SUN and TIME were
+  // computed once per cell with the same inputs, so not recomputing them
+  // changes no value.
+  __device__ __forceinline__ void rates_at(const RosParams& p, const CellState& s,
+                                           double T) const {
+    if (p.rates_given) return;
+    if (p.sun_mode == KPP_SUN_TOD) {
+      call_rates_sun<M>(s.TEMP, s.CFACTOR, update_sun(T), T, at(L::RCT));
+    } else if (M::RATES_USE_TIME) {
+      call_rates_sun<M>(s.TEMP, s.CFACTOR, s.SUN, T, at(L::RCT));
+    }
+  }
+
+  // Load a cell into this lane's slot; the set-up part of Rosenbrock()
+  // (int/rosenbrock.c:286-336) and RosenbrockIntegrator (:495-507).
+  __device__ void start(const RosParams& p, int cell, CellState& s) const {
+    const size_t ld = (size_t)p.ldc;
+    s.cell = cell;
+    s.TEMP = p.temp[cell];
+    s.CFACTOR = p.cfactor[cell];
+    s.SUN = p.sun ? p.sun[cell] : 0.0;
+    Acc Y = at(L::Y), F = at(L::FIX), R = at(L::RCT);
+    for (int i = 0; i < M::NVAR; i++) Y[i] = p.C[i * ld + cell];
+    for (int i = 0; i < M::NFIX; i++) F[i] = p.C[(M::NVAR + i) * ld + cell];
+    if (p.rates_given) {
+      for (int i = 0; i < M::NREACT; i++) R[i] = p.rconst[i * ld + cell];
+    } else {
+      call_rates_fixed<M>(s.TEMP, s.CFACTOR, R);
+      if (p.sun_mode == KPP_SUN_FIXED && !M::RATES_USE_TIME)
+        call_rates_sun<M>(s.TEMP, s.CFACTOR, s.SUN, 0.0, R);
+    }
+    s.Nfun = p.stat0[0]; s.Njac = p.stat0[1]; s.Nstp = p.stat0[2]; s.Nacc = p.stat0[3];
+    s.Nrej = p.stat0[4]; s.Ndec = p.stat0[5]; s.Nsol = p.stat0[6]; s.Nsng = p.stat0[7];
+    s.Tstart = p.tstart ? p.tstart[cell] : p.tstart0;
+    s.Tend = p.tend ? p.tend[cell] : p.tend0;
+    const double span = dabs(s.Tend - s.Tstart);
+    s.Hmax = (p.Hmax_user == 0.0) ? span : dmin(dabs(p.Hmax_user), span);
+    const double hs_user = p.hstart ? p.hstart[cell] : p.Hstart_user;
+    const double Hstart = (hs_user == 0.0) ? dmax(p.Hmin, KPP_DELTAMIN) : dmin(dabs(hs_user), span);
+    s.T = s.Tstart;
+    s.Hexit = 0.0;
+    s.H = dmin(Hstart, s.Hmax);
+    if (dabs(s.H) <= 10.0 * p.Roundoff) s.H = KPP_DELTAMIN;
+    s.Direction = (s.Tend >= s.Tstart) ? +1 : -1;
+    s.RejectLastH = false;
+    s.RejectMoreH = false;
+    s.newstep = true;
+  }
+
+  __device__ void finish(const RosParams& p, const CellState& s, int ierr, double Texit) const {
+    const size_t ld = (size_t)p.ldc;
+    const int cell = s.cell;
+    Acc Y = at(L::Y);
+    for (int i = 0; i < M::NVAR; i++) p.C[i * ld + cell] = Y[i];
+    p.ierr[cell] = ierr;
+    if (p.istat) {
+      int* q = p.istat;
+      q[0 * ld + cell] = s.Nfun; q[1 * ld + cell] = s.Njac; q[2 * ld + cell] = s.Nstp;
+      q[3 * ld + cell] = s.Nacc; q[4 * ld + cell] = s.Nrej; q[5 * ld + cell] = s.Ndec;
+      q[6 * ld + cell] = s.Nsol; q[7 * ld + cell] = s.Nsng;
+    }
+    if (p.rstat) {
+      p.rstat[cell] = Texit;
+      p.rstat[ld + cell] = s.Hexit;
+    }
+  }
+
+  // One step attempt.  Returns 0 while the cell continues, else its IERR
+  // (and the lane's cell is finished).
+  __device__ int attempt(const RosParams& p, CellState& s) const {
+    constexpr int NVAR = M::NVAR;
+    const RosCoef& m = p.coef;
+    const double Roundoff = p.Roundoff;
+    Acc Y = at(L::Y), F = at(L::FIX), R = at(L::RCT), Fcn0 = at(L::FCN0), Fcn = at(L::FCN),
+        dFdT = at(L::DFDT), Ynew = at(L::YNEW), K0 = at(L::K), G = at(L::G);
+    const bool skip_dfdt = p.dfdt_zero;
+    const double Dir = (double)s.Direction;
+
+    if (s.newstep) {
+      // time loop test and step guards (int/rosenbrock.c:510-524)
+      const bool more = ((s.Direction > 0) && ((s.T - s.Tend) + Roundoff <= 0.0)) ||
+                        ((s.Direction < 0) && ((s.Tend - s.T) + Roundoff <= 0.0));
+      if (!more) { finish(p, s, 1, s.T); return 1; }
+      if (s.Nstp > p.max_steps) { finish(p, s, -6, s.T); return -6; }
+      if (((s.T + 0.1 * s.H) == s.T) || (s.H <= Roundoff)) { finish(p, s, -7, s.T); return -7; }
+      s.Hexit = s.H;
+      s.H = dmin(s.H, dabs(s.Tend - s.T));
+      // Fcn0 = F(T, Y)
+      rates_at(p, s, s.T);
+      call_fun<M>(Y, F, R, Fcn0);
+      s.Nfun++;
+      // dF/dT by forward difference (int/rosenbrock.c:663-682)
+      if (!p.autonomous) {
+        s.Nfun++;
+        if (!skip_dfdt) {
+          const double Delta = sqrt(Roundoff) * dmax(KPP_DELTAMIN, dabs(s.T));
+          rates_at(p, s, s.T + Delta);
+          call_fun<M>(Y, F, R, dFdT);
+          const double inv = 1.0 / Delta;
+          for (int i = 0; i < NVAR; i++) {
+            const double d = dFdT[i] + (-1.0) * Fcn0[i];
+            dFdT[i] = (inv == 1.0) ? d : inv * d;
+          }
+        }
+      }
+      // Jac0 at (T, Y): counted here, evaluated inside every Ghimj build
+      // below from the same (T, Y, rates), hence with the same values.
+      s.Njac++;
+      s.newstep = false;
+    }
+
+    // ros_PrepareMatrix: Ghimj = -Jac0 + I/(H*gamma1), KppDecomp, halve H on
+    // an exactly zero diagonal, fail after 5 consecutive singular matrices.
+    rates_at(p, s, s.T);
+    int Nconsecutive = 0;
+    for (;;) {
+      const double ghinv = 1.0 / (Dir * s.H * m.Gamma[0]);
+      const int ising = call_jac_ghimj<M>(Y, F, R, ghinv, G);
+      s.Ndec++;
+      if (ising == 0) break;
+      s.Nsng++;
+      Nconsecutive++;
+      if (Nconsecutive <= 5) {
+        s.H = s.H * 0.5;
+      } else {
+        finish(p, s, -8, s.T);
+        return -8;
+      }
+    }
+    call_decomp<M>(G);
+
+    const double H = s.H;
+#pragma unroll
+    for (int istage = 1; istage <= S; istage++) {
+      Acc Ki{K0.b + (istage - 1) * NVAR * TILE};
+      // Fcn of this stage: output of the latest stage q<=istage with NewF,
+      // or Fcn0 (stage 1 copies Fcn0; int/rosenbrock.c:547-560)
+      bool from_fcn0 = true;
+#pragma unroll
+      for (int q = 2; q <= istage; q++)
+        if (m.NewF[q - 1]) from_fcn0 = false;
+      if (istage > 1 && m.NewF[istage - 1]) {
+        for (int i = 0; i < NVAR; i++) {
+          double y = Y[i];
+#pragma unroll
+          for (int j = 1; j <= istage - 1; j++) {
+            const double a = m.A[(istage - 1) * (istage - 2) / 2 + j - 1];
+            if (a != 0.0) y = y + a * K0[(j - 1) * NVAR + i];
+          }
+          Ynew[i] = y;
+        }
+        const double Tau = s.T + m.Alpha[istage - 1] * Dir * H;
+        rates_at(p, s, Tau);
+        call_fun<M>(Ynew, F, R, Fcn);
+        s.Nfun++;
+      }
+      Acc Fc = from_fcn0 ? Fcn0 : Fcn;
+      double HC[S > 1 ? S - 1 : 1];
+#pragma unroll
+      for (int j = 1; j <= istage - 1; j++)
+        HC[j - 1] = m.C[(istage - 1) * (istage - 2) / 2 + j - 1] / (Dir * H);
+      const double HG = Dir * H * m.Gamma[istage - 1];
+      const bool use_g = (!p.autonomous) && (m.Gamma[istage - 1] != 0.0) && !skip_dfdt &&
+                         (HG != 0.0);
+      for (int i = 0; i < NVAR; i++) {
+        double k = Fc[i];
+#pragma unroll
+        for (int j = 1; j <= istage - 1; j++)
+          if (HC[j - 1] != 0.0) k = k + HC[j - 1] * K0[(j - 1) * NVAR + i];
+        if (use_g) k = k + HG * dFdT[i];
+        Ki[i] = k;
+      }
+      call_solve<M>(G, Ki);
+      s.Nsol++;
+    }
+    // new solution, error estimate and ros_ErrorNorm (:582-591, :632-659)
+    double Err = 0.0;
+    for (int i = 0; i < NVAR; i++) {
+      const double y = Y[i];
+      double yn = y, ye = 0.0;
+#pragma unroll
+      for (int j = 1; j <= S; j++) {
+        const double kj = K0[(j - 1) * NVAR + i];
+        if (m.M[j - 1] != 0.0) yn = yn + m.M[j - 1] * kj;
+        if (m.E[j - 1] != 0.0) ye = ye + m.E[j - 1] * kj;
+      }
+      Ynew[i] = yn;
+      const double Ymax = dmax(dabs(y), dabs(yn));
+      const double Scale =
+          p.vector_tol ? (p.atol[i] + p.rtol[i] * Ymax) : (p.atol[0] + p.rtol[0] * Ymax);
+      Err = Err + (ye * ye) / (Scale * Scale);
+    }
+    Err = sqrt(Err / (double)NVAR);
+    // step size control (:594-617)
+    const double Fac = dmin(p.FacMax, dmax(p.FacMin, p.FacSafe / pow(Err, 1.0 / m.ELO)));
+    double Hnew = H * Fac;
+    s.Nstp++;
+    if ((Err <= 1.0) || (H <= p.Hmin)) {
+      s.Nacc++;
+      for (int i = 0; i < NVAR; i++) Y[i] = Ynew[i];
+      s.T += Dir * H;
+      Hnew = dmax(p.Hmin, dmin(Hnew, s.Hmax));
+      if (s.RejectLastH) Hnew = dmin(Hnew, H);
+      s.RejectLastH = false;
+      s.RejectMoreH = false;
+      s.H = Hnew;
+      s.newstep = true;
+    } else {
+      if (s.Nacc >= 1) s.Nrej++;
+      if (s.RejectMoreH) Hnew = H * p.FacRej;
+      s.RejectMoreH = s.RejectLastH;
+      s.RejectLastH = true;
+      s.H = Hnew;
+    }
+    return 0;
+  }
+};
+
+// Plain kernel: one lane per cell for the whole integration.
+template <class M, int S>
+__global__ __launch_bounds__(64) void ros_plain_kernel(RosParams p) {
+  using L = Layout<M, S>;
+  const int k = blockIdx.x * TILE + threadIdx.x;
+  if (k >= p.ncells) return;
+  Cell<M, S> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + threadIdx.x};
+  CellState s;
+  c.start(p, p.order ? p.order[k] : k, s);
+  while (c.attempt(p, s) == 0) {
+  }
+}
+
+// Persistent work-stealing kernel: a fixed grid of wavefronts; a lane whose
+// cell has ended takes the next cell of the queue.  Refills are batched per
+// wavefront: when at least refill_min lanes are idle (or none is active),
+// one lane claims as many queue entries as there are idle lanes with a
+// single atomic and the idle lanes take consecutive entries.  A wavefront
+// leaves once the queue is exhausted and none of its lanes is active, so
+// every wavefront reaches the exit.
+template <class M, int S>
+__global__ __launch_bounds__(64) void ros_persistent_kernel(RosParams p) {
+  using L = Layout<M, S>;
+  const int lane = threadIdx.x;
+  Cell<M, S> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + lane};
+  CellState s;
+  bool active = false;
+  bool exhausted = false;  // wavefront-uniform
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    const int n_idle = __popcll(idle);
+    if (!exhausted && n_idle > 0 && (n_idle >= p.refill_min || n_idle == TILE)) {
+      const int first = __ffsll((long long)idle) - 1;
+      int base = 0;
+      if (lane == first) base = atomicAdd(p.queue, n_idle);
+      base = __shfl(base, first);
+      if (base + n_idle >= p.ncells) exhausted = true;
+      if (!active) {
+        const int rank = __popcll(idle & ((1ull << lane) - 1ull));
+        const int k = base + rank;
+        if (k < p.ncells) {
+          c.start(p, p.order ? p.order[k] : k, s);
+          active = true;
+        }
+      }
+    }
+    if (__ballot(active) == 0) break;
+    if (active && c.attempt(p, s) != 0) active = false;
+  }
+}
+
+}  // namespace kpphip

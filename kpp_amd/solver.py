@@ -1,0 +1,304 @@
+"""Host-side binding of the batched Rosenbrock C ABI (include/kpphip.h).
+
+Mirrors KPP's integrator interface for many cells at once:
+``rosenbrock(...)`` takes the arguments of ``Rosenbrock(Y, Tstart, Tend,
+AbsTol, RelTol, ode_Fun, ode_Jac, RPAR, IPAR)`` (int/rosenbrock.c:137-446)
+with Y/Tstart/Tend per cell, keeps the IPAR/RPAR index meanings, and
+returns Rosenbrock's per-cell IERR codes plus the IPAR[10..17] and
+RPAR[10..11] outputs per cell.  Device memory is handled with torch tensors;
+the arithmetic runs in the HIP library, which must be present: there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libkpphip.so"
+GEN = PKG / "_gen"
+
+SUN_FIXED = 0
+SUN_TOD = 1
+PLAIN = 0
+PERSISTENT = 1
+
+# IPAR[3] codes of int/rosenbrock.c:218-225
+ROS2, ROS3, ROS4, RODAS3, RODAS4 = 1, 2, 3, 4, 5
+METHOD_STAGES = {ROS2: 2, ROS3: 3, ROS4: 4, RODAS3: 4, RODAS4: 6}
+
+IERR_TEXT = {
+    1: "success",
+    -1: "Improper value for maximal no of steps",
+    -2: "Selected Rosenbrock method not implemented",
+    -3: "Hmin/Hmax/Hstart must be positive",
+    -4: "FacMin/FacMax/FacRej must be positive",
+    -5: "Improper tolerance values",
+    -6: "No of steps exceeds maximum bound",
+    -7: "Step size too small (T + H/10 = T) or H < Roundoff",
+    -8: "Matrix is repeatedly singular",
+}
+
+
+class KppError(RuntimeError):
+    pass
+
+
+class _Batch(ctypes.Structure):
+    _fields_ = [
+        ("ncells", ctypes.c_int), ("ld", ctypes.c_int),
+        ("C", ctypes.c_void_p), ("temp", ctypes.c_void_p), ("cfactor", ctypes.c_void_p),
+        ("sun", ctypes.c_void_p), ("sun_mode", ctypes.c_int),
+        ("tstart", ctypes.c_void_p), ("tend", ctypes.c_void_p),
+        ("tstart0", ctypes.c_double), ("tend0", ctypes.c_double),
+        ("hstart", ctypes.c_void_p), ("rconst", ctypes.c_void_p),
+        ("ierr", ctypes.c_void_p), ("istat", ctypes.c_void_p), ("rstat", ctypes.c_void_p),
+        ("order", ctypes.c_void_p),
+    ]
+
+
+class _Launch(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int), ("nblocks", ctypes.c_int), ("refill_min", ctypes.c_int)]
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Load libkpphip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise KppError(f"{LIB_PATH} is missing: build it with python -m kpp_amd.build "
+                       "(there is no CPU fallback for the solver)")
+    L = ctypes.CDLL(str(LIB_PATH))
+    vp, ci, cd, cs = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_char_p
+    L.kpp_mech_count.restype = ci
+    L.kpp_mech_name.restype = cs
+    L.kpp_mech_name.argtypes = [ci]
+    L.kpp_mech_info.argtypes = [cs, vp]
+    L.kpp_mech_species.restype = cs
+    L.kpp_mech_species.argtypes = [cs, ci]
+    L.kpp_mech_lu_pattern.argtypes = [cs, vp, vp, vp]
+    L.kpp_check_params.argtypes = [vp, vp, vp, vp, ci]
+    L.kpp_workspace_bytes.restype = ctypes.c_size_t
+    L.kpp_workspace_bytes.argtypes = [cs, ci, ctypes.POINTER(_Launch), ci]
+    L.kpp_rosenbrock_dev.argtypes = [cs, ctypes.POINTER(_Batch), vp, vp, vp, vp,
+                                     ctypes.POINTER(_Launch), vp, ctypes.c_size_t, vp]
+    for f in ("kpp_fun_dev", "kpp_jac_sp_dev"):
+        getattr(L, f).argtypes = [cs, ci, ci, vp, vp, vp, vp, vp]
+    L.kpp_decomp_dev.argtypes = [cs, ci, ci, vp, vp, vp]
+    L.kpp_solve_dev.argtypes = [cs, ci, ci, vp, vp, vp]
+    L.kpp_rates_dev.argtypes = [cs, ci, ci, vp, vp, vp, vp, vp, vp]
+    L.kpp_ctx_create.argtypes = [ctypes.POINTER(vp), cs, ci, ci, ci]
+    L.kpp_ctx_destroy.argtypes = [vp]
+    L.kpp_set_state.argtypes = [vp, vp, vp, vp, vp]
+    L.kpp_rosenbrock_batch.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    L.kpp_get_state.argtypes = [vp, vp, vp, vp, vp]
+    L.kpp_ctx_set_tol.argtypes = [vp, vp, vp, ci]
+    L.INTEGRATE_BATCH.argtypes = [vp, cd, cd]
+    _lib = L
+    return L
+
+
+@dataclass(frozen=True)
+class Mechanism:
+    name: str
+    nvar: int
+    nfix: int
+    nspec: int
+    nreact: int
+    nonzero: int
+    lu_nonzero: int
+    rates_use_time: bool
+    n_sun_rates: int
+    species: tuple
+    stats: dict
+
+    @property
+    def var_species(self):
+        return self.species[: self.nvar]
+
+
+def mechanisms() -> list[str]:
+    L = library()
+    return [L.kpp_mech_name(i).decode() for i in range(L.kpp_mech_count())]
+
+
+def mechanism(name: str) -> Mechanism:
+    L = library()
+    out = np.zeros(8, np.int32)
+    r = L.kpp_mech_info(name.encode(), out.ctypes.data)
+    if r != 0:
+        raise KppError(f"unknown mechanism {name!r} (compiled: {mechanisms()})")
+    sp = tuple(L.kpp_mech_species(name.encode(), k).decode() for k in range(out[2]))
+    stats_path = GEN / f"{name}_stats.json"
+    stats = json.loads(stats_path.read_text()) if stats_path.exists() else {}
+    return Mechanism(name, *map(int, out[:6]), bool(out[6]), int(out[7]), sp, stats)
+
+
+def lu_pattern(name: str):
+    m = mechanism(name)
+    icol = np.zeros(m.lu_nonzero, np.int32)
+    crow = np.zeros(m.nvar + 1, np.int32)
+    diag = np.zeros(m.nvar + 1, np.int32)
+    library().kpp_mech_lu_pattern(name.encode(), icol.ctypes.data, crow.ctypes.data,
+                                  diag.ctypes.data)
+    return icol, crow, diag
+
+
+def check_params(IPAR, RPAR, atol, rtol, nvar) -> int:
+    IPAR = np.ascontiguousarray(IPAR, np.int32)
+    RPAR = np.ascontiguousarray(RPAR, np.float64)
+    atol = np.ascontiguousarray(atol, np.float64)
+    rtol = np.ascontiguousarray(rtol, np.float64)
+    return library().kpp_check_params(IPAR.ctypes.data, RPAR.ctypes.data, atol.ctypes.data,
+                                      rtol.ctypes.data, nvar)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def default_ipar(method: int, scalar_tol: bool = True, autonomous: bool = False,
+                 max_steps: int = 0):
+    IPAR = np.zeros(20, np.int32)
+    IPAR[0] = 1 if autonomous else 0
+    IPAR[1] = 1 if scalar_tol else 0
+    IPAR[2] = max_steps
+    IPAR[3] = method
+    return IPAR
+
+
+class Workspace:
+    """Device workspace for kpp_rosenbrock_dev, grown on demand."""
+
+    def __init__(self, device="cuda"):
+        self.device = device
+        self.buf = None
+
+    def get(self, nbytes: int):
+        import torch
+        if self.buf is None or self.buf.numel() < nbytes:
+            self.buf = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+        return self.buf
+
+
+_default_ws = {}
+
+
+def workspace_bytes(mech: str, method: int, ncells: int, variant=PLAIN, nblocks=0,
+                    refill_min=0) -> int:
+    l = _Launch(variant, nblocks, refill_min)
+    return int(library().kpp_workspace_bytes(mech.encode(), method, ctypes.byref(l), ncells))
+
+
+def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, RPAR=None,
+               sun=None, sun_mode=SUN_FIXED, hstart=None, rconst=None, order=None,
+               ierr=None, istat=None, rstat=None, variant=PLAIN, nblocks=0, refill_min=0,
+               workspace=None, stream=None):
+    """Batched Rosenbrock on device tensors (float64 SoA, C is [NSPEC, ncells]).
+
+    tstart/tend: per-cell tensors or floats.  atol/rtol: host sequences of
+    NVAR entries (IPAR[1]==0) or 1.  Returns (code, ierr, istat, rstat) with
+    code 0 or Rosenbrock's parameter error; launches asynchronously on
+    ``stream`` (torch's current stream by default).
+    """
+    import torch
+    L = library()
+    n = int(C.shape[1])
+    dev = C.device
+    if C.dtype != torch.float64 or not C.is_contiguous():
+        raise KppError("C must be a contiguous float64 tensor [NSPEC, ncells]")
+    if ierr is None:
+        ierr = torch.empty(n, dtype=torch.int32, device=dev)
+    if istat is None:
+        istat = torch.empty((8, n), dtype=torch.int32, device=dev)
+    if rstat is None:
+        rstat = torch.empty((2, n), dtype=torch.float64, device=dev)
+    b = _Batch()
+    b.ncells = n
+    b.ld = n
+    b.C = C.data_ptr()
+    b.temp = temp.data_ptr()
+    b.cfactor = cfactor.data_ptr()
+    b.sun = sun.data_ptr() if sun is not None else None
+    b.sun_mode = sun_mode
+    if torch.is_tensor(tstart):
+        b.tstart = tstart.data_ptr()
+    else:
+        b.tstart0 = float(tstart)
+    if torch.is_tensor(tend):
+        b.tend = tend.data_ptr()
+    else:
+        b.tend0 = float(tend)
+    b.hstart = hstart.data_ptr() if hstart is not None else None
+    b.rconst = rconst.data_ptr() if rconst is not None else None
+    b.ierr = ierr.data_ptr()
+    b.istat = istat.data_ptr()
+    b.rstat = rstat.data_ptr()
+    b.order = order.data_ptr() if order is not None else None
+    IPAR = np.ascontiguousarray(IPAR, np.int32)
+    RPAR = np.zeros(20) if RPAR is None else np.ascontiguousarray(RPAR, np.float64)
+    atol = np.ascontiguousarray(np.atleast_1d(atol), np.float64)
+    rtol = np.ascontiguousarray(np.atleast_1d(rtol), np.float64)
+    l = _Launch(variant, nblocks, refill_min)
+    nbytes = L.kpp_workspace_bytes(mech.encode(), int(IPAR[3]), ctypes.byref(l), n)
+    if workspace is None:
+        workspace = _default_ws.setdefault(str(dev), Workspace(dev))
+    ws = workspace.get(max(nbytes, 256))
+    code = L.kpp_rosenbrock_dev(mech.encode(), ctypes.byref(b), IPAR.ctypes.data,
+                                RPAR.ctypes.data, atol.ctypes.data, rtol.ctypes.data,
+                                ctypes.byref(l), ctypes.c_void_p(ws.data_ptr()),
+                                ctypes.c_size_t(ws.numel()), _stream_ptr(stream))
+    if code <= -100:
+        raise KppError(f"kpp_rosenbrock_dev failed with {code}")
+    return code, ierr, istat, rstat
+
+
+def _comp(fn, *args):
+    r = fn(*args)
+    if r != 0:
+        raise KppError(f"{fn.__name__} failed with {r}")
+
+
+def fun(mech, V, F, RCT, out, stream=None):
+    n = V.shape[1]
+    _comp(library().kpp_fun_dev, mech.encode(), n, n, _ptr(V), _ptr(F), _ptr(RCT), _ptr(out),
+          _stream_ptr(stream))
+
+
+def jac_sp(mech, V, F, RCT, out, stream=None):
+    n = V.shape[1]
+    _comp(library().kpp_jac_sp_dev, mech.encode(), n, n, _ptr(V), _ptr(F), _ptr(RCT), _ptr(out),
+          _stream_ptr(stream))
+
+
+def decomp(mech, JVS, ising, stream=None):
+    n = JVS.shape[1]
+    _comp(library().kpp_decomp_dev, mech.encode(), n, n, _ptr(JVS), _ptr(ising),
+          _stream_ptr(stream))
+
+
+def solve(mech, JVS, X, stream=None):
+    n = X.shape[1]
+    _comp(library().kpp_solve_dev, mech.encode(), n, n, _ptr(JVS), _ptr(X), _stream_ptr(stream))
+
+
+def rates(mech, temp, cfactor, sun, time, out, stream=None):
+    n = temp.shape[0]
+    _comp(library().kpp_rates_dev, mech.encode(), n, n, _ptr(temp), _ptr(cfactor), _ptr(sun),
+          _ptr(time), _ptr(out), _stream_ptr(stream))

@@ -1,0 +1,613 @@
+/* ros_ref.c — CPU oracle for the batched Rosenbrock solver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker or the reported CPU baseline, never as the product path.
+ *
+ * A reentrant restatement of KPP's C Rosenbrock path, one cell per call:
+ *   Rosenbrock             int/rosenbrock.c:137-446 (parameter defaults,
+ *                          checks and error codes, incl. the IPAR[3]==0
+ *                          quirk of :303-310)
+ *   RosenbrockIntegrator   int/rosenbrock.c:450-627
+ *   ros_ErrorNorm          int/rosenbrock.c:632-659
+ *   ros_FunTimeDerivative  int/rosenbrock.c:663-682
+ *   ros_PrepareMatrix      int/rosenbrock.c:686-737
+ *   Ros2/Ros3/Ros4/Rodas3/Rodas3_1/Rodas4 tables  int/rosenbrock.c:779-1222
+ *   Fun/Jac/Decomp/Solve templates               int/rosenbrock.c:1227-1296
+ *   KppDecomp              util/sutil.c:2-23
+ *   WCOPY/WAXPY/WSCAL/WLAMCH  util/blas.c:18-180
+ *   Update_SUN             util/UpdateSun.c:1-20
+ * The model functions (Fun, Jac_SP, KppSolve, Update_RCONST, rate laws of
+ * util/UserRateLaws.c) come from the mechanism compiler's C output, included
+ * through KPP_MODEL_HEADER.
+ *
+ * Global state of the reference (statistics counters, TIME, SUN, RCONST,
+ * FIX) lives in a per-cell context, so cells run in parallel under OpenMP.
+ * Two documented extensions of the reference's driver: SUN can be held
+ * fixed per cell (sun_mode 0: Update_SUN is bypassed, as the fixed-sun
+ * workloads specify), and the rate constants can be supplied by the caller
+ * (rconst != NULL) instead of computed, to compare arithmetic bit for bit
+ * with a device whose libm differs.
+ *
+ * Parity pins: see tests/test_oracle_*.py (golden vectors from the
+ * reference's documentation, order conditions, closed forms, dense LU,
+ * scipy reference integrations).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include KPP_MODEL_HEADER
+
+#define MAX(a, b) (((a) >= (b)) ? (a) : (b))
+#define MIN(b, c) (((b) < (c)) ? (b) : (c))
+#define ABS(x) (((x) >= 0) ? (x) : (-x))
+#define SQRT(d) (pow((d), 0.5))
+#define ZERO 0.0
+#define ONE 1.0
+#define HALF 0.5
+#define DeltaMin 1.0e-6
+#define Smax 6
+
+typedef struct {
+  int Nfun, Njac, Nstp, Nacc, Nrej, Ndec, Nsol, Nsng;
+} ros_stats;
+
+typedef struct {
+  double TEMP, CFACTOR, SUN;
+  int sun_mode; /* 0: SUN fixed per cell; 1: SUN from Update_SUN(TIME) */
+  double FIX[KPP_NFIX > 0 ? KPP_NFIX : 1];
+  double RCONST[KPP_NREACT];
+  int rates_given;
+  ros_stats st;
+} ros_cell;
+
+typedef struct {
+  int S;
+  double A[Smax * (Smax - 1) / 2], C[Smax * (Smax - 1) / 2];
+  double M[Smax], E[Smax], Alpha[Smax], Gamma[Smax], ELO;
+  char NewF[Smax];
+} ros_method;
+
+/* ---------- util/blas.c ---------- */
+static void WCOPY(int N, const double* X, double* Y) {
+  for (int i = 0; i < N; i++) Y[i] = X[i];
+}
+static void WAXPY(int N, double Alpha, const double* X, double* Y) {
+  if (Alpha == ZERO) return;
+  for (int i = 0; i < N; i++) Y[i] = Y[i] + Alpha * X[i];
+}
+static void WSCAL(int N, double Alpha, double* X) {
+  if (Alpha == ONE) return;
+  if (Alpha == -ONE) {
+    for (int i = 0; i < N; i++) X[i] = -X[i];
+  } else if (Alpha == ZERO) {
+    for (int i = 0; i < N; i++) X[i] = ZERO;
+  } else {
+    for (int i = 0; i < N; i++) X[i] = Alpha * X[i];
+  }
+}
+static double WLAMCH_ADD(double A, double B) { return A + B; }
+double ros_ref_wlamch(void) {
+  /* halving loop of util/blas.c:150-180 */
+  volatile double Eps = pow(HALF, 16), Suma;
+  int i;
+  for (i = 17; i <= 80; i++) {
+    Eps = Eps * HALF;
+    Suma = WLAMCH_ADD(ONE, Eps);
+    if (Suma <= ONE) break;
+  }
+  if (i == 80) return 2.2e-16;
+  return Eps * 2.0;
+}
+
+/* ---------- util/UpdateSun.c ---------- */
+double ros_ref_update_sun(double TIME) {
+  double SunRise = 4.5, SunSet = 19.5, Thour, Tlocal, Ttmp;
+  const double PI = 3.14159265358979;
+  Thour = TIME / 3600.0;
+  Tlocal = Thour - ((int)Thour / 24) * 24;
+  if ((Tlocal >= SunRise) && (Tlocal <= SunSet)) {
+    Ttmp = (2.0 * Tlocal - SunRise - SunSet) / (SunSet - SunRise);
+    if (Ttmp > 0) Ttmp = Ttmp * Ttmp;
+    else Ttmp = -Ttmp * Ttmp;
+    return (1.0 + cos(PI * Ttmp)) / 2.0;
+  }
+  return 0.0;
+}
+
+/* ---------- util/sutil.c KppDecomp ---------- */
+static int KppDecomp(double* JVS) {
+  double W[KPP_NVAR];
+  double a;
+  int k, kk, j, jj;
+  for (k = 0; k < KPP_NVAR; k++) {
+    if (JVS[kpp_LU_DIAG[k]] == 0.0) return k + 1;
+    for (kk = kpp_LU_CROW[k]; kk < kpp_LU_CROW[k + 1]; kk++) W[kpp_LU_ICOL[kk]] = JVS[kk];
+    for (kk = kpp_LU_CROW[k]; kk < kpp_LU_DIAG[k]; kk++) {
+      j = kpp_LU_ICOL[kk];
+      a = -W[j] / JVS[kpp_LU_DIAG[j]];
+      W[j] = -a;
+      for (jj = kpp_LU_DIAG[j] + 1; jj < kpp_LU_CROW[j + 1]; jj++) W[kpp_LU_ICOL[jj]] += a * JVS[jj];
+    }
+    for (kk = kpp_LU_CROW[k]; kk < kpp_LU_CROW[k + 1]; kk++) JVS[kk] = W[kpp_LU_ICOL[kk]];
+  }
+  return 0;
+}
+
+/* ---------- templates (int/rosenbrock.c:1227-1296) ---------- */
+static void update_rates(ros_cell* c, double T) {
+  if (c->rates_given) return;
+  if (c->sun_mode == 1) c->SUN = ros_ref_update_sun(T);
+  kpp_Update_RCONST(c->TEMP, c->SUN, c->CFACTOR, T, c->RCONST);
+}
+static void FunTemplate(ros_cell* c, double T, const double* Y, double* Ydot) {
+  update_rates(c, T);
+  kpp_Fun(Y, c->FIX, c->RCONST, Ydot);
+  c->st.Nfun++;
+}
+static void JacTemplate(ros_cell* c, double T, const double* Y, double* Jcb) {
+  update_rates(c, T);
+  kpp_Jac_SP(Y, c->FIX, c->RCONST, Jcb);
+  c->st.Njac++;
+}
+
+/* ---------- method tables (int/rosenbrock.c:779-1222) ---------- */
+static void Ros2(ros_method* m) {
+  double g = 1.70710678118655;
+  m->S = 2;
+  m->A[0] = 1.0 / g;
+  m->C[0] = (-2.0) / g;
+  m->NewF[0] = 1; m->NewF[1] = 1;
+  m->M[0] = (3.0) / (2.0 * g);
+  m->M[1] = (1.0) / (2.0 * g);
+  m->E[0] = 1.0 / (2.0 * g);
+  m->E[1] = 1.0 / (2.0 * g);
+  m->ELO = 2.0;
+  m->Alpha[0] = 0.0; m->Alpha[1] = 1.0;
+  m->Gamma[0] = g; m->Gamma[1] = -g;
+}
+static void Ros3(ros_method* m) {
+  m->S = 3;
+  m->A[0] = 1.0; m->A[1] = 1.0; m->A[2] = 0.0;
+  m->C[0] = -1.0156171083877702091975600115545;
+  m->C[1] = 4.0759956452537699824805835358067;
+  m->C[2] = 9.2076794298330791242156818474003;
+  m->NewF[0] = 1; m->NewF[1] = 1; m->NewF[2] = 0;
+  m->M[0] = 1.0;
+  m->M[1] = 6.1697947043828245592553615689730;
+  m->M[2] = -0.4277225654321857332623837380651;
+  m->E[0] = 0.5;
+  m->E[1] = -2.9079558716805469821718236208017;
+  m->E[2] = 0.2235406989781156962736090927619;
+  m->ELO = 3.0;
+  m->Alpha[0] = 0.0;
+  m->Alpha[1] = 0.43586652150845899941601945119356;
+  m->Alpha[2] = 0.43586652150845899941601945119356;
+  m->Gamma[0] = 0.43586652150845899941601945119356;
+  m->Gamma[1] = 0.24291996454816804366592249683314;
+  m->Gamma[2] = 2.1851380027664058511513169485832;
+}
+static void Ros4(ros_method* m) {
+  m->S = 4;
+  m->A[0] = 0.2000000000000000e+01;
+  m->A[1] = 0.1867943637803922e+01;
+  m->A[2] = 0.2344449711399156;
+  m->A[3] = m->A[1];
+  m->A[4] = m->A[2];
+  m->A[5] = 0.0;
+  m->C[0] = -0.7137615036412310e+01;
+  m->C[1] = 0.2580708087951457e+01;
+  m->C[2] = 0.6515950076447975;
+  m->C[3] = -0.2137148994382534e+01;
+  m->C[4] = -0.3214669691237626;
+  m->C[5] = -0.6949742501781779;
+  m->NewF[0] = 1; m->NewF[1] = 1; m->NewF[2] = 1; m->NewF[3] = 0;
+  m->M[0] = 0.2255570073418735e+01;
+  m->M[1] = 0.2870493262186792;
+  m->M[2] = 0.4353179431840180;
+  m->M[3] = 0.1093502252409163e+01;
+  m->E[0] = -0.2815431932141155;
+  m->E[1] = -0.7276199124938920e-01;
+  m->E[2] = -0.1082196201495311;
+  m->E[3] = -0.1093502252409163e+01;
+  m->ELO = 4.0;
+  m->Alpha[0] = 0.0;
+  m->Alpha[1] = 0.1145640000000000e+01;
+  m->Alpha[2] = 0.6552168638155900;
+  m->Alpha[3] = m->Alpha[2];
+  m->Gamma[0] = 0.5728200000000000;
+  m->Gamma[1] = -0.1769193891319233e+01;
+  m->Gamma[2] = 0.7592633437920482;
+  m->Gamma[3] = -0.1049021087100450;
+}
+static void Rodas3(ros_method* m) {
+  m->S = 4;
+  m->A[0] = 0.0; m->A[1] = 2.0; m->A[2] = 0.0; m->A[3] = 2.0; m->A[4] = 0.0; m->A[5] = 1.0;
+  m->C[0] = 4.0; m->C[1] = 1.0; m->C[2] = -1.0; m->C[3] = 1.0; m->C[4] = -1.0;
+  m->C[5] = -2.66666666666667;
+  m->NewF[0] = 1; m->NewF[1] = 0; m->NewF[2] = 1; m->NewF[3] = 1;
+  m->M[0] = 2.0; m->M[1] = 0.0; m->M[2] = 1.0; m->M[3] = 1.0;
+  m->E[0] = 0.0; m->E[1] = 0.0; m->E[2] = 0.0; m->E[3] = 1.0;
+  m->ELO = 3.0;
+  m->Alpha[0] = 0.0; m->Alpha[1] = 0.0; m->Alpha[2] = 1.0; m->Alpha[3] = 1.0;
+  m->Gamma[0] = 0.5; m->Gamma[1] = 1.5; m->Gamma[2] = 0.0; m->Gamma[3] = 0.0;
+}
+static void Rodas3_1(ros_method* m) {
+  m->S = 4;
+  m->A[0] = 0.000000000000000; m->A[1] = 0.646601929740551; m->A[2] = 0.409567801987914;
+  m->A[3] = 0.646601929740551; m->A[4] = 0.409567801987914; m->A[5] = 1.000000000000000;
+  m->C[0] = 4.198495621784201; m->C[1] = 3.711590161613010; m->C[2] = -1.787771994729384;
+  m->C[3] = 4.458898153216104; m->C[4] = -2.024095448516552; m->C[5] = -2.626700600119396;
+  m->NewF[0] = 1; m->NewF[1] = 0; m->NewF[2] = 1; m->NewF[3] = 1;
+  m->M[0] = 0.646601929740551; m->M[1] = 0.409567801987914; m->M[2] = 1.0; m->M[3] = 1.0;
+  m->E[0] = 0.0; m->E[1] = 0.0; m->E[2] = 0.0; m->E[3] = 1.0;
+  m->ELO = 3.0;
+  m->Alpha[0] = 0.0; m->Alpha[1] = 0.0; m->Alpha[2] = 1.0; m->Alpha[3] = 1.0;
+  m->Gamma[0] = 0.515; m->Gamma[1] = 1.628546001287715; m->Gamma[2] = 0.0; m->Gamma[3] = 0.0;
+}
+static void Rodas4(ros_method* m) {
+  m->S = 6;
+  m->Alpha[0] = 0.000; m->Alpha[1] = 0.386; m->Alpha[2] = 0.210;
+  m->Alpha[3] = 0.630; m->Alpha[4] = 1.000; m->Alpha[5] = 1.000;
+  m->Gamma[0] = 0.2500000000000000; m->Gamma[1] = -0.1043000000000000;
+  m->Gamma[2] = 0.1035000000000000; m->Gamma[3] = -0.3620000000000023e-01;
+  m->Gamma[4] = 0.0; m->Gamma[5] = 0.0;
+  m->A[0] = 0.1544000000000000e+01;
+  m->A[1] = 0.9466785280815826;
+  m->A[2] = 0.2557011698983284;
+  m->A[3] = 0.3314825187068521e+01;
+  m->A[4] = 0.2896124015972201e+01;
+  m->A[5] = 0.9986419139977817;
+  m->A[6] = 0.1221224509226641e+01;
+  m->A[7] = 0.6019134481288629e+01;
+  m->A[8] = 0.1253708332932087e+02;
+  m->A[9] = -0.6878860361058950;
+  m->A[10] = m->A[6]; m->A[11] = m->A[7]; m->A[12] = m->A[8]; m->A[13] = m->A[9];
+  m->A[14] = 1.0;
+  m->C[0] = -0.5668800000000000e+01;
+  m->C[1] = -0.2430093356833875e+01;
+  m->C[2] = -0.2063599157091915;
+  m->C[3] = -0.1073529058151375;
+  m->C[4] = -0.9594562251023355e+01;
+  m->C[5] = -0.2047028614809616e+02;
+  m->C[6] = 0.7496443313967647e+01;
+  m->C[7] = -0.1024680431464352e+02;
+  m->C[8] = -0.3399990352819905e+02;
+  m->C[9] = 0.1170890893206160e+02;
+  m->C[10] = 0.8083246795921522e+01;
+  m->C[11] = -0.7981132988064893e+01;
+  m->C[12] = -0.3152159432874371e+02;
+  m->C[13] = 0.1631930543123136e+02;
+  m->C[14] = -0.6058818238834054e+01;
+  m->M[0] = m->A[6]; m->M[1] = m->A[7]; m->M[2] = m->A[8]; m->M[3] = m->A[9];
+  m->M[4] = 1.0; m->M[5] = 1.0;
+  for (int i = 0; i < 5; i++) m->E[i] = 0.0;
+  m->E[5] = 1.0;
+  for (int i = 0; i < 6; i++) m->NewF[i] = 1;
+  m->ELO = 4.0;
+}
+
+/* Fill a method table; code as IPAR[3] (1..5, 7). Returns 0 if unknown. */
+int ros_ref_method(int code, int* S, double* A, double* C, double* M, double* E, double* Alpha,
+                   double* Gamma, double* ELO, int* NewF) {
+  ros_method m;
+  memset(&m, 0, sizeof m);
+  switch (code) {
+    case 1: Ros2(&m); break;
+    case 2: Ros3(&m); break;
+    case 3: Ros4(&m); break;
+    case 4: Rodas3(&m); break;
+    case 5: Rodas4(&m); break;
+    case 7: Rodas3_1(&m); break;
+    default: return 0;
+  }
+  *S = m.S;
+  for (int i = 0; i < Smax * (Smax - 1) / 2; i++) { A[i] = m.A[i]; C[i] = m.C[i]; }
+  for (int i = 0; i < Smax; i++) {
+    M[i] = m.M[i]; E[i] = m.E[i]; Alpha[i] = m.Alpha[i]; Gamma[i] = m.Gamma[i];
+    NewF[i] = m.NewF[i];
+  }
+  *ELO = m.ELO;
+  return 1;
+}
+
+/* ---------- ros_ErrorNorm (:632-659) ---------- */
+static double ros_ErrorNorm(const double* Y, const double* Ynew, const double* Yerr,
+                            const double* AbsTol, const double* RelTol, int VectorTol) {
+  double Err = ZERO, Scale, Ymax;
+  for (int i = 0; i < KPP_NVAR; i++) {
+    Ymax = MAX(ABS(Y[i]), ABS(Ynew[i]));
+    if (VectorTol) Scale = AbsTol[i] + RelTol[i] * Ymax;
+    else Scale = AbsTol[0] + RelTol[0] * Ymax;
+    Err = Err + (Yerr[i] * Yerr[i]) / (Scale * Scale);
+  }
+  return SQRT(Err / (double)KPP_NVAR);
+}
+
+/* ---------- ros_FunTimeDerivative (:663-682) ---------- */
+static void ros_FunTimeDerivative(ros_cell* c, double T, double Roundoff, const double* Y,
+                                  const double* Fcn0, double* dFdT) {
+  double Delta = SQRT(Roundoff) * MAX(DeltaMin, ABS(T));
+  FunTemplate(c, T + Delta, Y, dFdT);
+  WAXPY(KPP_NVAR, (-ONE), Fcn0, dFdT);
+  WSCAL(KPP_NVAR, (ONE / Delta), dFdT);
+}
+
+/* ---------- ros_PrepareMatrix (:686-737) ---------- */
+static int ros_PrepareMatrix(ros_cell* c, double* H, int Direction, double gam,
+                             const double* Jac0, double* Ghimj) {
+  int Nconsecutive = 0, ising;
+  double ghinv;
+  for (;;) {
+    WCOPY(KPP_LU_NONZERO, Jac0, Ghimj);
+    WSCAL(KPP_LU_NONZERO, (-ONE), Ghimj);
+    ghinv = ONE / (Direction * (*H) * gam);
+    for (int i = 0; i < KPP_NVAR; i++) Ghimj[kpp_LU_DIAG[i]] = Ghimj[kpp_LU_DIAG[i]] + ghinv;
+    ising = KppDecomp(Ghimj);
+    c->st.Ndec++;
+    if (ising == 0) return 0;
+    c->st.Nsng++;
+    Nconsecutive++;
+    if (Nconsecutive <= 5) *H = (*H) * HALF;
+    else return 1;
+  }
+}
+
+/* ---------- RosenbrockIntegrator (:450-627) ---------- */
+static int RosenbrockIntegrator(ros_cell* c, double* Y, double Tstart, double Tend,
+                                const double* AbsTol, const double* RelTol, const ros_method* m,
+                                int Autonomous, int VectorTol, int Max_no_steps, double Roundoff,
+                                double Hmin, double Hmax, double Hstart, double FacMin,
+                                double FacMax, double FacRej, double FacSafe, double* Texit,
+                                double* Hexit) {
+  double Ynew[KPP_NVAR], Fcn0[KPP_NVAR], Fcn[KPP_NVAR], dFdT[KPP_NVAR];
+  double Jac0[KPP_LU_NONZERO], Ghimj[KPP_LU_NONZERO];
+  double K[KPP_NVAR * Smax];
+  double H, T, Hnew, HC, HG, Fac, Tau, Err, Yerr[KPP_NVAR];
+  int Direction, ioffset, j, istage;
+  int RejectLastH, RejectMoreH;
+  const int ros_S = m->S;
+
+  T = Tstart;
+  *Hexit = 0.0;
+  H = MIN(Hstart, Hmax);
+  if (ABS(H) <= 10.0 * Roundoff) H = DeltaMin;
+  Direction = (Tend >= Tstart) ? +1 : -1;
+  RejectLastH = 0;
+  RejectMoreH = 0;
+
+  while (((Direction > 0) && ((T - Tend) + Roundoff <= ZERO)) ||
+         ((Direction < 0) && ((Tend - T) + Roundoff <= ZERO))) {
+    if (c->st.Nstp > Max_no_steps) { *Texit = T; return -6; }
+    if (((T + 0.1 * H) == T) || (H <= Roundoff)) { *Texit = T; return -7; }
+    *Hexit = H;
+    H = MIN(H, ABS(Tend - T));
+    FunTemplate(c, T, Y, Fcn0);
+    if (!Autonomous) ros_FunTimeDerivative(c, T, Roundoff, Y, Fcn0, dFdT);
+    JacTemplate(c, T, Y, Jac0);
+    for (;;) {
+      if (ros_PrepareMatrix(c, &H, Direction, m->Gamma[0], Jac0, Ghimj)) {
+        *Texit = T;
+        return -8;
+      }
+      for (istage = 1; istage <= ros_S; istage++) {
+        ioffset = KPP_NVAR * (istage - 1);
+        if (istage == 1) {
+          WCOPY(KPP_NVAR, Fcn0, Fcn);
+        } else if (m->NewF[istage - 1]) {
+          WCOPY(KPP_NVAR, Y, Ynew);
+          for (j = 1; j <= istage - 1; j++)
+            WAXPY(KPP_NVAR, m->A[(istage - 1) * (istage - 2) / 2 + j - 1], &K[KPP_NVAR * (j - 1)],
+                  Ynew);
+          Tau = T + m->Alpha[istage - 1] * Direction * H;
+          FunTemplate(c, Tau, Ynew, Fcn);
+        }
+        WCOPY(KPP_NVAR, Fcn, &K[ioffset]);
+        for (j = 1; j <= istage - 1; j++) {
+          HC = m->C[(istage - 1) * (istage - 2) / 2 + j - 1] / (Direction * H);
+          WAXPY(KPP_NVAR, HC, &K[KPP_NVAR * (j - 1)], &K[ioffset]);
+        }
+        if ((!Autonomous) && (m->Gamma[istage - 1])) {
+          HG = Direction * H * m->Gamma[istage - 1];
+          WAXPY(KPP_NVAR, HG, dFdT, &K[ioffset]);
+        }
+        kpp_KppSolve(Ghimj, &K[ioffset]);
+        c->st.Nsol++;
+      }
+      WCOPY(KPP_NVAR, Y, Ynew);
+      for (j = 1; j <= ros_S; j++) WAXPY(KPP_NVAR, m->M[j - 1], &K[KPP_NVAR * (j - 1)], Ynew);
+      WSCAL(KPP_NVAR, ZERO, Yerr);
+      for (j = 1; j <= ros_S; j++) WAXPY(KPP_NVAR, m->E[j - 1], &K[KPP_NVAR * (j - 1)], Yerr);
+      Err = ros_ErrorNorm(Y, Ynew, Yerr, AbsTol, RelTol, VectorTol);
+      Fac = MIN(FacMax, MAX(FacMin, FacSafe / pow(Err, ONE / m->ELO)));
+      Hnew = H * Fac;
+      c->st.Nstp++;
+      if ((Err <= ONE) || (H <= Hmin)) {
+        c->st.Nacc++;
+        WCOPY(KPP_NVAR, Ynew, Y);
+        T += Direction * H;
+        Hnew = MAX(Hmin, MIN(Hnew, Hmax));
+        if (RejectLastH) Hnew = MIN(Hnew, H);
+        RejectLastH = 0;
+        RejectMoreH = 0;
+        H = Hnew;
+        break;
+      } else {
+        if (c->st.Nacc >= 1) c->st.Nrej++;
+        if (RejectMoreH) Hnew = H * FacRej;
+        RejectMoreH = RejectLastH;
+        RejectLastH = 1;
+        H = Hnew;
+      }
+    }
+  }
+  *Texit = T;
+  return 1;
+}
+
+/* ---------- Rosenbrock (:137-446) ---------- */
+static int Rosenbrock(ros_cell* c, double* Y, double Tstart, double Tend, const double* AbsTol,
+                      const double* RelTol, double* RPAR, int* IPAR) {
+  ros_method m;
+  int Max_no_steps, IERR, UplimTol, Method, Autonomous, VectorTol;
+  double Roundoff, FacMin, FacMax, FacRej, FacSafe, Hmin, Hmax, Hstart, Hexit = 0, Texit = 0;
+
+  c->st.Nfun = IPAR[10];
+  c->st.Njac = IPAR[11];
+  c->st.Nstp = IPAR[12];
+  c->st.Nacc = IPAR[13];
+  c->st.Nrej = IPAR[14];
+  c->st.Ndec = IPAR[15];
+  c->st.Nsol = IPAR[16];
+  c->st.Nsng = IPAR[17];
+  Autonomous = !(IPAR[0] == 0);
+  if (IPAR[1] == 0) { VectorTol = 1; UplimTol = KPP_NVAR; }
+  else { VectorTol = 0; UplimTol = 1; }
+  Max_no_steps = (IPAR[2] == 0) ? 100000 : IPAR[2];
+  if (Max_no_steps < 0) return -1;
+  Method = (IPAR[3] == 0) ? 3 : IPAR[3];
+  if ((IPAR[3] < 1) || (IPAR[3] > 5)) return -2;
+  Roundoff = ros_ref_wlamch();
+  Hmin = RPAR[0];
+  if (RPAR[0] < ZERO) return -3;
+  if (RPAR[1] == ZERO) Hmax = ABS(Tend - Tstart);
+  else Hmax = MIN(ABS(RPAR[1]), ABS(Tend - Tstart));
+  if (RPAR[1] < ZERO) return -3;
+  if (RPAR[2] == ZERO) Hstart = MAX(Hmin, DeltaMin);
+  else Hstart = MIN(ABS(RPAR[2]), ABS(Tend - Tstart));
+  if (RPAR[2] < ZERO) return -3;
+  FacMin = (RPAR[3] == ZERO) ? 0.2 : RPAR[3];
+  if (RPAR[3] < ZERO) return -4;
+  FacMax = (RPAR[4] == ZERO) ? 6.0 : RPAR[4];
+  if (RPAR[4] < ZERO) return -4;
+  FacRej = (RPAR[5] == ZERO) ? 0.1 : RPAR[5];
+  if (RPAR[5] < ZERO) return -4;
+  FacSafe = (RPAR[6] == ZERO) ? 0.9 : RPAR[6];
+  if (RPAR[6] < ZERO) return -4;
+  for (int i = 0; i < UplimTol; i++)
+    if ((AbsTol[i] <= ZERO) || (RelTol[i] <= 10.0 * Roundoff) || (RelTol[i] >= ONE)) return -5;
+  memset(&m, 0, sizeof m);
+  switch (Method) {
+    case 1: Ros2(&m); break;
+    case 2: Ros3(&m); break;
+    case 3: Ros4(&m); break;
+    case 4: Rodas3(&m); break;
+    case 5: Rodas4(&m); break;
+    case 7: Rodas3_1(&m); break;
+    default: return -2;
+  }
+  IERR = RosenbrockIntegrator(c, Y, Tstart, Tend, AbsTol, RelTol, &m, Autonomous, VectorTol,
+                              Max_no_steps, Roundoff, Hmin, Hmax, Hstart, FacMin, FacMax, FacRej,
+                              FacSafe, &Texit, &Hexit);
+  IPAR[10] = c->st.Nfun;
+  IPAR[11] = c->st.Njac;
+  IPAR[12] = c->st.Nstp;
+  IPAR[13] = c->st.Nacc;
+  IPAR[14] = c->st.Nrej;
+  IPAR[15] = c->st.Ndec;
+  IPAR[16] = c->st.Nsol;
+  IPAR[17] = c->st.Nsng;
+  RPAR[10] = Texit;
+  RPAR[11] = Hexit;
+  return IERR;
+}
+
+/* ======================= exported entry points ======================= */
+int ros_ref_sizes(int* out) {
+  out[0] = KPP_NVAR;
+  out[1] = KPP_NFIX;
+  out[2] = KPP_NSPEC;
+  out[3] = KPP_NREACT;
+  out[4] = KPP_NONZERO;
+  out[5] = KPP_LU_NONZERO;
+  return 6;
+}
+const char* ros_ref_root(void) { return KPP_ROOT; }
+void ros_ref_lu_pattern(int* icol, int* crow, int* diag) {
+  for (int i = 0; i < KPP_LU_NONZERO; i++) icol[i] = kpp_LU_ICOL[i];
+  for (int i = 0; i <= KPP_NVAR; i++) { crow[i] = kpp_LU_CROW[i]; diag[i] = kpp_LU_DIAG[i]; }
+}
+const char* ros_ref_species_name(int k) { return (k >= 0 && k < KPP_NSPEC) ? kpp_SPC_NAMES[k] : 0; }
+void ros_ref_initialize(double* C) { kpp_Initialize(C); }
+void ros_ref_rconst(double TEMP, double SUN, double CFACTOR, double TIME, double* RCONST) {
+  kpp_Update_RCONST(TEMP, SUN, CFACTOR, TIME, RCONST);
+}
+void ros_ref_fun(const double* V, const double* F, const double* RCT, double* Vdot) {
+  kpp_Fun(V, F, RCT, Vdot);
+}
+void ros_ref_jac_sp(const double* V, const double* F, const double* RCT, double* JVS) {
+  kpp_Jac_SP(V, F, RCT, JVS);
+}
+int ros_ref_decomp(double* JVS) { return KppDecomp(JVS); }
+void ros_ref_solve(const double* JVS, double* X) { kpp_KppSolve(JVS, X); }
+/* Ghimj build + decomposition of ros_PrepareMatrix for one fixed H */
+int ros_ref_prepare(const double* Jac0, double H, double gam, double* Ghimj) {
+  ros_cell c;
+  memset(&c, 0, sizeof c);
+  double h = H;
+  int r = ros_PrepareMatrix(&c, &h, +1, gam, Jac0, Ghimj);
+  return r ? -1 : c.st.Nsng;
+}
+double ros_ref_error_norm(const double* Y, const double* Ynew, const double* Yerr,
+                          const double* AbsTol, const double* RelTol, int VectorTol) {
+  return ros_ErrorNorm(Y, Ynew, Yerr, AbsTol, RelTol, VectorTol);
+}
+
+/* Batched integration over SoA concentrations C[s*ncells + c] (NSPEC rows).
+ * temp/cfactor/sun: per-cell arrays; sun_mode 0 = fixed SUN, 1 = Update_SUN.
+ * tstart/tend: per-cell arrays.  atol/rtol: NVAR entries (IPAR[1]==0) or 1.
+ * hstart: per-cell starting step (overrides RPAR[2]) or NULL.
+ * rconst: per-cell rate constants (SoA NREACT x ncells) or NULL.
+ * ierr[c], istat[8*ncells] (Nfun..Nsng, SoA), rstat[2*ncells] (Texit, Hexit).
+ */
+void ros_ref_batch(int ncells, double* C, const double* temp, const double* cfactor,
+                   const double* sun, int sun_mode, const double* tstart, const double* tend,
+                   const double* atol, const double* rtol, const int* IPAR_in,
+                   const double* RPAR_in, const double* hstart, const double* rconst, int* ierr,
+                   int* istat, double* rstat, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+  for (int cidx = 0; cidx < ncells; cidx++) {
+    ros_cell c;
+    double Y[KPP_NVAR];
+    int IPAR[20];
+    double RPAR[20];
+    memset(&c, 0, sizeof c);
+    for (int i = 0; i < 20; i++) { IPAR[i] = IPAR_in[i]; RPAR[i] = RPAR_in[i]; }
+    if (hstart) RPAR[2] = hstart[cidx];
+    c.TEMP = temp[cidx];
+    c.CFACTOR = cfactor[cidx];
+    c.SUN = sun ? sun[cidx] : 0.0;
+    c.sun_mode = sun_mode;
+    for (int i = 0; i < KPP_NVAR; i++) Y[i] = C[(size_t)i * ncells + cidx];
+    for (int i = 0; i < KPP_NFIX; i++) c.FIX[i] = C[(size_t)(KPP_NVAR + i) * ncells + cidx];
+    if (rconst) {
+      c.rates_given = 1;
+      for (int i = 0; i < KPP_NREACT; i++) c.RCONST[i] = rconst[(size_t)i * ncells + cidx];
+    }
+    int e = Rosenbrock(&c, Y, tstart[cidx], tend[cidx], atol, rtol, RPAR, IPAR);
+    for (int i = 0; i < KPP_NVAR; i++) C[(size_t)i * ncells + cidx] = Y[i];
+    ierr[cidx] = e;
+    if (istat)
+      for (int i = 0; i < 8; i++) istat[(size_t)i * ncells + cidx] = IPAR[10 + i];
+    if (rstat) {
+      rstat[cidx] = RPAR[10];
+      rstat[(size_t)ncells + cidx] = RPAR[11];
+    }
+  }
+}
+
+int ros_ref_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
