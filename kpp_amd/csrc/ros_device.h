@@ -72,6 +72,12 @@ __device__ __forceinline__ double update_sun(double TIME) {
 __device__ __forceinline__ double dmax(double a, double b) { return (a >= b) ? a : b; }
 __device__ __forceinline__ double dmin(double b, double c) { return (b < c) ? b : c; }
 __device__ __forceinline__ double dabs(double x) { return (x >= 0) ? x : -x; }
+// ABS(a-b) as the unparenthesised macro of int/rosenbrock.c:3 expands it:
+// ((a-b) >= 0) ? (a-b) : (-a-b).  For a < b (backward integration) this is
+// -a-b, which makes Rosenbrock() stop with -7 at once; reproduced on purpose.
+__device__ __forceinline__ double abs_diff_macro(double a, double b) {
+  return ((a - b) >= 0) ? (a - b) : (-a - b);
+}
 
 // Model calls kept out of line: one copy of each large straight-line routine
 // per kernel bounds instruction-cache footprint.
@@ -153,7 +159,7 @@ struct Cell {
     s.Nrej = p.stat0[4]; s.Ndec = p.stat0[5]; s.Nsol = p.stat0[6]; s.Nsng = p.stat0[7];
     s.Tstart = p.tstart ? p.tstart[cell] : p.tstart0;
     s.Tend = p.tend ? p.tend[cell] : p.tend0;
-    const double span = dabs(s.Tend - s.Tstart);
+    const double span = abs_diff_macro(s.Tend, s.Tstart);
     s.Hmax = (p.Hmax_user == 0.0) ? span : dmin(dabs(p.Hmax_user), span);
     const double hs_user = p.hstart ? p.hstart[cell] : p.Hstart_user;
     const double Hstart = (hs_user == 0.0) ? dmax(p.Hmin, KPP_DELTAMIN) : dmin(dabs(hs_user), span);
@@ -204,7 +210,7 @@ struct Cell {
       if (s.Nstp > p.max_steps) { finish(p, s, -6, s.T); return -6; }
       if (((s.T + 0.1 * s.H) == s.T) || (s.H <= Roundoff)) { finish(p, s, -7, s.T); return -7; }
       s.Hexit = s.H;
-      s.H = dmin(s.H, dabs(s.Tend - s.T));
+      s.H = dmin(s.H, abs_diff_macro(s.Tend, s.T));
       // Fcn0 = F(T, Y)
       rates_at(p, s, s.T);
       call_fun<M>(Y, F, R, Fcn0);
