@@ -1,0 +1,199 @@
+"""Benchmark: cells/s for a 1 h integration of saprc99 cells (RTOL 1e-3).
+
+One step = one batched integration of the whole synthetic batch (default:
+1,048,576 saprc99 cells, 50 % night / 50 % day with SUN ~ U(0.2, 1),
+Rodas3, FP64), inputs resident in HBM; the concentrations are reset from a
+pristine device copy at the start of each step.  ``value`` is the tuned path
+(persistent work-stealing kernel); the plain one-cell-per-lane kernel is
+timed on the same cells and reported beside it, and the CPU oracle (this
+project's C restatement of KPP's int/rosenbrock.c, OpenMP over cells) is
+timed on a bounded sample of the same cells on the host's cores.
+
+Roofline: the solver is bound by HBM bandwidth.  Algorithmic bytes per step
+attempt (one unit) follow the streamed working set of each attempt:
+  first attempt of a step: 8*[NNZ + 2NNZ + 2NNZ + S*NNZ + NF1*(2NVAR+NFIX+NREACT)
+                             + (S^2/2+3S+6)*NVAR]
+  each retry:              the same without the Jacobian write and NF2 Fun calls
+with NF1/NF2 the Fun evaluations of a first attempt / retry (Rodas3: 4 / 2).
+Attempt counts come from the solver's own per-cell counters (Nstp, Njac).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_work(mech, method, istat):
+    """(bytes, flops) of all step attempts of a batch from its counters."""
+    from kpp_amd import solver
+    st = mech.stats
+    nnz, nv, nf, nr = mech.lu_nonzero, mech.nvar, mech.nfix, mech.nreact
+    S = solver.METHOD_STAGES[method]
+    nf1, nf2 = {solver.ROS2: (3, 1), solver.ROS3: (3, 1), solver.ROS4: (4, 2),
+                solver.RODAS3: (4, 2), solver.RODAS4: (7, 5)}[method]
+    vec = (S * S / 2 + 3 * S + 6) * nv
+    fun_io = 2 * nv + nf + nr
+    b_first = 8 * (nnz + 2 * nnz + 2 * nnz + S * nnz + nf1 * fun_io + vec)
+    b_retry = 8 * (2 * nnz + 2 * nnz + S * nnz + nf2 * fun_io + vec)
+    fl = st["flops"]
+    f_vec = (S * S + 3 * S + 6) * nv
+    f_first = fl["jac"] + (nnz + nv) + fl["decomp"] + S * fl["solve"] + nf1 * fl["fun"] + f_vec
+    f_retry = (nnz + nv) + fl["decomp"] + S * fl["solve"] + nf2 * fl["fun"] + f_vec
+    nstp = istat[2].astype(np.int64)
+    njac = istat[1].astype(np.int64)
+    first = int(njac.sum())
+    retry = int((nstp - njac).sum())
+    return (first * b_first + retry * b_retry, first * f_first + retry * f_retry,
+            first + retry, {"bytes_first_attempt": b_first, "bytes_retry": b_retry})
+
+
+def cpu_baseline(wl_fn, ncells_full, seed, target_s=12.0):
+    """Time the oracle on a bounded sample of the same workload."""
+    from oracle import ref
+    threads = len(os.sched_getaffinity(0))
+    cal = wl_fn(ncells=256, seed=seed)
+    t = time.time()
+    ref.integrate(cal, nthreads=threads, fast=True)
+    rate = 256 / max(time.time() - t, 1e-3)
+    n = int(min(ncells_full, max(1024, rate * target_s)))
+    # an evenly spaced subset of the batch's cells
+    cells = np.linspace(0, ncells_full - 1, n).astype(np.int64)
+    b = wl_fn(seed=seed, cells=cells)
+    t = time.time()
+    out = ref.integrate(b, nthreads=threads, fast=True)
+    dt = time.time() - t
+    return {"value": n / dt, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"{n} evenly spaced cells of the same {ncells_full}-cell batch, "
+                      f"oracle ros_ref (gcc -O2 -march=x86-64-v3, OpenMP dynamic schedule), "
+                      f"{dt:.1f} s", "ierr_ok": bool((out["ierr"] == 1).all())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="saprc99_uneven")
+    ap.add_argument("--ncells", type=int, default=1048576)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--nblocks", type=int, default=0)
+    ap.add_argument("--refill-min", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): cells are "
+              "independent and shard without communication; run one process per GPU.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from kpp_amd import solver, workloads
+
+    wl_fn = workloads.WORKLOADS[args.workload]
+    t0 = time.time()
+    b = wl_fn(ncells=args.ncells, seed=args.seed)
+    mech = solver.mechanism(b.mech)
+    log(f"[bench] workload {args.workload}: {b.ncells} cells generated in {time.time()-t0:.1f}s")
+    dev = torch.device("cuda:0")
+    d = b.to_device(dev)
+    C0 = d["C"].clone()
+    C = d["C"]
+    n = b.ncells
+    ierr = torch.empty(n, dtype=torch.int32, device=dev)
+    istat = torch.empty((8, n), dtype=torch.int32, device=dev)
+    rstat = torch.empty((2, n), dtype=torch.float64, device=dev)
+    ws = solver.Workspace(dev)
+
+    def step(variant):
+        C.copy_(C0)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        code, _, _, _ = solver.rosenbrock(
+            b.mech, C, d["temp"], d["cfactor"], tstart=d["tstart"], tend=d["tend"],
+            atol=b.atol, rtol=b.rtol, IPAR=b.IPAR, RPAR=b.RPAR, sun=d["sun"],
+            sun_mode=b.sun_mode, ierr=ierr, istat=istat, rstat=rstat, variant=variant,
+            nblocks=args.nblocks if variant == solver.PERSISTENT else 0,
+            refill_min=args.refill_min, workspace=ws)
+        ev1.record()
+        assert code == 0, code
+        return ev0, ev1
+
+    def timed(variant, steps, warmup, name):
+        for _ in range(warmup):
+            step(variant)
+        torch.cuda.synchronize()
+        evs = []
+        t = time.perf_counter()
+        for k in range(steps):
+            evs.append(step(variant))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        kern = [a.elapsed_time(e) / 1e3 for a, e in evs]
+        ok = bool((ierr == 1).all().item())
+        log(f"[bench] {name}: {steps} steps, {wall / steps * 1e3:.1f} ms/step, "
+            f"kernel {np.mean(kern) * 1e3:.1f} ms, all IERR=1: {ok}")
+        return wall, float(np.mean(kern)), ok
+
+    wall_t, kern_t, ok_t = timed(solver.PERSISTENT, args.steps, args.warmup, "tuned (persistent)")
+    ist = istat.cpu().numpy()
+    nbytes, nflops, attempts, bterms = algorithmic_work(mech, b.method, ist)
+    plain = None
+    if not args.no_plain:
+        wall_p, kern_p, ok_p = timed(solver.PLAIN, max(1, args.steps), max(0, args.warmup), "plain")
+        plain = {"value": n / (wall_p / max(1, args.steps)), "kernel_ms": kern_p * 1e3,
+                 "all_ierr_1": ok_p}
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] timing the CPU oracle on a sample ...")
+        cpu = cpu_baseline(wl_fn, n, args.seed)
+        log(f"[bench] cpu baseline {cpu['value']:.0f} cells/s on {cpu['cores']} threads")
+    value = n / (wall_t / args.steps)
+    achieved = nbytes / kern_t / 1e9
+    line = {
+        "metric": "cells/s integrated 1 h (saprc99, RTOL 1e-3): tuned vs plain GPU vs CPU",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_t / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "mechanism": b.mech, "ncells": n,
+                   "method": "Rodas3" if b.method == solver.RODAS3 else int(b.method),
+                   "rtol": float(b.rtol[0]), "atol": float(b.atol[0]), "integrate_s": 3600,
+                   "path": "persistent work-stealing kernel", "all_ierr_1": ok_t,
+                   "attempts_per_cell": attempts / n},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "ros_persistent_kernel<kppm_saprc99,4>",
+                     "kernel_ms": kern_t * 1e3, "algorithmic_bytes": nbytes,
+                     "fp64_tflops": nflops / kern_t / 1e12,
+                     "fp64_frac": nflops / kern_t / 1e12 / FP64_PEAK_TFLOPS, **bterms},
+        "plain": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
