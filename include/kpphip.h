@@ -84,7 +84,9 @@ typedef struct {
 typedef struct {
   int variant;    /* KPP_VARIANT_PLAIN / KPP_VARIANT_PERSISTENT */
   int nblocks;    /* persistent: wavefronts in the grid (0 = library default) */
-  int refill_min; /* persistent: refill when this many lanes are idle (0 = default) */
+  int refill_min; /* persistent: refill when this many lanes are idle (0 = default 1) */
+  int lanes;      /* cells per wavefront, 1..64 (0 = library default: 64, fewer for
+                     batches too small to fill the device) */
 } kpp_launch;
 
 /* Device bytes of workspace needed for a launch. */

@@ -47,6 +47,9 @@ HIP_FLAGS = [
     # packed FP32 instructions stay off in every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-function", "-Wno-unused-variable",
+    # register budget for >= 4 resident wavefronts per SIMD: the integrator
+    # is latency bound and twice as fast at 4 waves/SIMD as at 1-2
+    "-DKPPHIP_WAVES_PER_EU=4",
 ]
 
 

@@ -158,6 +158,7 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct Geometry {
   int nblocks;
   int refill_min;
+  int lanes;
   size_t tol_bytes, tile_bytes, total;
 };
 
@@ -165,21 +166,30 @@ bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry
   const int rows = m->rows(S);
   if (rows <= 0) return false;
   const int variant = l ? l->variant : KPP_VARIANT_PLAIN;
+  if (variant != KPP_VARIANT_PLAIN && variant != KPP_VARIANT_PERSISTENT) return false;
+  int cus = device_cus();
+  if (cus <= 0) cus = 256;
+  int occ = m->occupancy(variant, S);
+  if (occ <= 0) occ = 4;
+  const int resident = cus * occ;  // wavefronts the device holds at once
+  int lanes = (l && l->lanes > 0) ? l->lanes : 0;
+  if (lanes > 64) return false;
+  if (lanes == 0) {
+    // 64 cells per wavefront unless that leaves resident wavefront slots
+    // empty; then fewer (down to 16), so more wavefronts share the latency
+    lanes = 64;
+    while (lanes > 16 && (long)(ncells + lanes - 1) / lanes < resident) lanes /= 2;
+  }
+  g.lanes = lanes;
   if (variant == KPP_VARIANT_PERSISTENT) {
-    int nb = (l && l->nblocks > 0) ? l->nblocks : 0;
-    if (nb == 0) {
-      const int cus = device_cus();
-      nb = (cus > 0 ? cus : 256) * 4;
-    }
-    const int need = (ncells + 63) / 64;
+    int nb = (l && l->nblocks > 0) ? l->nblocks : resident;
+    const int need = (ncells + lanes - 1) / lanes;
     if (nb > need) nb = need > 0 ? need : 1;
     g.nblocks = nb;
-    g.refill_min = (l && l->refill_min > 0) ? l->refill_min : 16;
-  } else if (variant == KPP_VARIANT_PLAIN) {
-    g.nblocks = (ncells + 63) / 64;
-    g.refill_min = 0;
+    g.refill_min = (l && l->refill_min > 0) ? l->refill_min : 1;
   } else {
-    return false;
+    g.nblocks = (ncells + lanes - 1) / lanes;
+    g.refill_min = 0;
   }
   g.tol_bytes = align256(2 * (size_t)m->nvar * sizeof(double));
   g.tile_bytes = (size_t)rows * 64 * sizeof(double);
@@ -328,6 +338,7 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
   p.queue = (int*)ws;
   p.scratch = (double*)(ws + kHead + g.tol_bytes);
   p.refill_min = g.refill_min;
+  p.lanes = g.lanes;
   const int variant = l ? l->variant : KPP_VARIANT_PLAIN;
   if (m->launch(p, variant, g.nblocks, st) != hipSuccess) return KPP_E_HIP;
   return 0;
@@ -404,7 +415,7 @@ int kpp_ctx_create(kpp_ctx** out, const char* mech, int ncells, int method, int 
   c->rtol.assign(1, 1.0e-3);
   c->vector_tol = 0;
   const size_t n = (size_t)ncells;
-  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0};
+  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0};
   c->ws_bytes = kpp_workspace_bytes(mech, method, &l, ncells);
   if (c->ws_bytes == 0) { ctx_free(c); return -2; }
   bool ok = hipMalloc(&c->C, sizeof(double) * n * m->nspec) == hipSuccess &&
@@ -461,7 +472,7 @@ static int ctx_run(kpp_ctx* c, const int* IPAR, const double* RPAR, const double
   b.ierr = c->ierr;
   b.istat = c->istat;
   b.rstat = c->rstat;
-  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0};
+  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0};
   int r = kpp_rosenbrock_dev(c->m->name, &b, IPAR, RPAR, atol, rtol, &l, c->ws, c->ws_bytes,
                              nullptr);
   if (hipDeviceSynchronize() != hipSuccess) return KPP_E_HIP;

@@ -49,6 +49,7 @@ struct RosParams {
   double* scratch;         // tiles
   int* queue;              // persistent kernel: next-cell counter (starts at 0)
   int refill_min;          // persistent kernel: refill when >= this many lanes idle
+  int lanes;               // cells per wavefront (1..64)
   RosCoef coef;
 };
 

@@ -86,6 +86,26 @@ hipError_t launch_mech(const RosParams& p, int variant, int nblocks, hipStream_t
   return hipErrorInvalidValue;
 }
 
+// Resident wavefronts per CU of a kernel (its register/LDS limits).
+template <class M, int S>
+int occupancy_s(int variant) {
+  int n = 0;
+  hipError_t e = (variant == KPP_VARIANT_PERSISTENT)
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_persistent_kernel<M, S>, TILE, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_plain_kernel<M, S>, TILE, 0);
+  return e == hipSuccess ? n : 0;
+}
+template <class M>
+int occupancy_mech(int variant, int S) {
+  switch (S) {
+    case 2: return occupancy_s<M, 2>(variant);
+    case 3: return occupancy_s<M, 3>(variant);
+    case 4: return occupancy_s<M, 4>(variant);
+    case 6: return occupancy_s<M, 6>(variant);
+  }
+  return 0;
+}
+
 template <class M>
 int rows_mech(int S) {
   switch (S) {
@@ -133,6 +153,7 @@ MechOps make_ops(const char* name) {
   o.lu_crow = D::crow();
   o.lu_diag = D::diag();
   o.rows = rows_mech<M>;
+  o.occupancy = occupancy_mech<M>;
   o.launch = launch_mech<M>;
   o.component = component_mech<M>;
   return o;

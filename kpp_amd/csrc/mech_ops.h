@@ -23,6 +23,7 @@ struct MechOps {
   const char* const* species;
   const int *lu_icol, *lu_crow, *lu_diag;
   int (*rows)(int S);  // tile rows per lane for S stages
+  int (*occupancy)(int variant, int S);  // resident wavefronts per CU (0 if unknown)
   hipError_t (*launch)(const RosParams& p, int variant, int nblocks, hipStream_t st);
   hipError_t (*component)(int which, int n, size_t ld, const double* a, const double* b,
                           const double* c, const double* d, double* out, int* iout,

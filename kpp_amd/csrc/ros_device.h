@@ -28,14 +28,43 @@
 
 #include "kpphip_params.h"
 
+// Build knobs (set by the autotuned build, kpp_amd/build.py):
+//  KPPHIP_WAVES_PER_EU  minimum resident wavefronts per SIMD the register
+//                       allocator must allow (0: compiler default)
+//  KPPHIP_MODEL_INLINE  1: inline the model routines into the kernels;
+//                       0: call them out of line (one copy per kernel)
+#ifndef KPPHIP_WAVES_PER_EU
+#define KPPHIP_WAVES_PER_EU 0
+#endif
+#ifndef KPPHIP_MODEL_INLINE
+#define KPPHIP_MODEL_INLINE 0
+#endif
+#if KPPHIP_WAVES_PER_EU > 0
+#define KPPHIP_KERNEL_ATTR __attribute__((amdgpu_waves_per_eu(KPPHIP_WAVES_PER_EU)))
+#else
+#define KPPHIP_KERNEL_ATTR
+#endif
+#if KPPHIP_MODEL_INLINE
+#define KPPHIP_MODEL_CALL __device__ __forceinline__
+#else
+#define KPPHIP_MODEL_CALL __device__ __noinline__ __attribute__((used))
+#endif
+
 namespace kpphip {
 
 constexpr int TILE = 64;
 
+// Tiles live in global memory.  Saying so in the pointer type matters for
+// the out-of-line model routines: through a generic pointer every access is
+// a flat load/store, whose results may return out of order, so the compiler
+// has to drain all outstanding loads before using any of them; global loads
+// return in order and can be kept in flight back to back.
+using gdouble = __attribute__((address_space(1))) double;
+
 // Element accessor for a lane inside a 64-lane tile.
 struct Acc {
-  double* __restrict__ b;
-  __device__ __forceinline__ double& operator[](int i) const { return b[i * TILE]; }
+  gdouble* __restrict__ b;
+  __device__ __forceinline__ gdouble& operator[](int i) const { return b[i * TILE]; }
 };
 
 // Row layout of a tile for mechanism M and S stages.
@@ -80,31 +109,36 @@ __device__ __forceinline__ double abs_diff_macro(double a, double b) {
 }
 
 // Model calls kept out of line: one copy of each large straight-line routine
-// per kernel bounds instruction-cache footprint.
+// per kernel bounds instruction-cache footprint and register pressure.  The
+// arrays a routine touches are distinct rows of the lane's tile; declaring
+// them __restrict__ lets the compiler keep values in registers across stores
+// and issue loads early (otherwise every store to X in KppSolve would order
+// all later loads of G behind it).
+using dptr = gdouble* __restrict__;
 template <class M>
-__device__ __noinline__ __attribute__((used)) void call_fun(Acc V, Acc F, Acc R, Acc out) {
-  M::Fun(V, F, R, out);
+KPPHIP_MODEL_CALL void call_fun(dptr V, dptr F, dptr R, dptr out) {
+  M::Fun(Acc{V}, Acc{F}, Acc{R}, Acc{out});
 }
 template <class M>
-__device__ __noinline__ __attribute__((used)) int call_jac_ghimj(Acc V, Acc F, Acc R, double ghinv, Acc G) {
-  return M::JacGhimj(V, F, R, ghinv, G);
+KPPHIP_MODEL_CALL int call_jac_ghimj(dptr V, dptr F, dptr R, double ghinv, dptr G) {
+  return M::JacGhimj(Acc{V}, Acc{F}, Acc{R}, ghinv, Acc{G});
 }
 template <class M>
-__device__ __noinline__ __attribute__((used)) void call_decomp(Acc G) {
-  M::KppDecomp(G);
+KPPHIP_MODEL_CALL void call_decomp(dptr G) {
+  M::KppDecomp(Acc{G});
 }
 template <class M>
-__device__ __noinline__ __attribute__((used)) void call_solve(Acc G, Acc X) {
-  M::KppSolve(G, X);
+KPPHIP_MODEL_CALL void call_solve(dptr G, dptr X) {
+  M::KppSolve(Acc{G}, Acc{X});
 }
 template <class M>
-__device__ __noinline__ __attribute__((used)) void call_rates_sun(double TEMP, double CFACTOR, double SUN, double TIME,
-                                            Acc R) {
-  M::rates_sun(TEMP, CFACTOR, SUN, TIME, R);
+KPPHIP_MODEL_CALL void call_rates_sun(double TEMP, double CFACTOR, double SUN, double TIME,
+                                      dptr R) {
+  M::rates_sun(TEMP, CFACTOR, SUN, TIME, Acc{R});
 }
 template <class M>
-__device__ __noinline__ __attribute__((used)) void call_rates_fixed(double TEMP, double CFACTOR, Acc R) {
-  M::rates_fixed(TEMP, CFACTOR, R);
+KPPHIP_MODEL_CALL void call_rates_fixed(double TEMP, double CFACTOR, dptr R) {
+  M::rates_fixed(TEMP, CFACTOR, Acc{R});
 }
 
 // Per-lane integration state between attempts.
@@ -121,7 +155,7 @@ template <class M, int S>
 struct Cell {
   using L = Layout<M, S>;
   double* base;  // lane base pointer inside its tile
-  __device__ __forceinline__ Acc at(int row) const { return Acc{base + row * TILE}; }
+  __device__ __forceinline__ Acc at(int row) const { return Acc{(gdouble*)(base + row * TILE)}; }
 
   // Rate constants at time T (int/rosenbrock.c:1257-1296 call Update_SUN and
   // Update_RCONST before every Fun/Jac).  Rates free of SUN and TIME were
@@ -131,9 +165,9 @@ struct Cell {
                                            double T) const {
     if (p.rates_given) return;
     if (p.sun_mode == KPP_SUN_TOD) {
-      call_rates_sun<M>(s.TEMP, s.CFACTOR, update_sun(T), T, at(L::RCT));
+      call_rates_sun<M>(s.TEMP, s.CFACTOR, update_sun(T), T, at(L::RCT).b);
     } else if (M::RATES_USE_TIME) {
-      call_rates_sun<M>(s.TEMP, s.CFACTOR, s.SUN, T, at(L::RCT));
+      call_rates_sun<M>(s.TEMP, s.CFACTOR, s.SUN, T, at(L::RCT).b);
     }
   }
 
@@ -151,9 +185,9 @@ struct Cell {
     if (p.rates_given) {
       for (int i = 0; i < M::NREACT; i++) R[i] = p.rconst[i * ld + cell];
     } else {
-      call_rates_fixed<M>(s.TEMP, s.CFACTOR, R);
+      call_rates_fixed<M>(s.TEMP, s.CFACTOR, R.b);
       if (p.sun_mode == KPP_SUN_FIXED && !M::RATES_USE_TIME)
-        call_rates_sun<M>(s.TEMP, s.CFACTOR, s.SUN, 0.0, R);
+        call_rates_sun<M>(s.TEMP, s.CFACTOR, s.SUN, 0.0, R.b);
     }
     s.Nfun = p.stat0[0]; s.Njac = p.stat0[1]; s.Nstp = p.stat0[2]; s.Nacc = p.stat0[3];
     s.Nrej = p.stat0[4]; s.Ndec = p.stat0[5]; s.Nsol = p.stat0[6]; s.Nsng = p.stat0[7];
@@ -213,7 +247,7 @@ struct Cell {
       s.H = dmin(s.H, abs_diff_macro(s.Tend, s.T));
       // Fcn0 = F(T, Y)
       rates_at(p, s, s.T);
-      call_fun<M>(Y, F, R, Fcn0);
+      call_fun<M>(Y.b, F.b, R.b, Fcn0.b);
       s.Nfun++;
       // dF/dT by forward difference (int/rosenbrock.c:663-682)
       if (!p.autonomous) {
@@ -221,7 +255,7 @@ struct Cell {
         if (!skip_dfdt) {
           const double Delta = sqrt(Roundoff) * dmax(KPP_DELTAMIN, dabs(s.T));
           rates_at(p, s, s.T + Delta);
-          call_fun<M>(Y, F, R, dFdT);
+          call_fun<M>(Y.b, F.b, R.b, dFdT.b);
           const double inv = 1.0 / Delta;
           for (int i = 0; i < NVAR; i++) {
             const double d = dFdT[i] + (-1.0) * Fcn0[i];
@@ -241,7 +275,7 @@ struct Cell {
     int Nconsecutive = 0;
     for (;;) {
       const double ghinv = 1.0 / (Dir * s.H * m.Gamma[0]);
-      const int ising = call_jac_ghimj<M>(Y, F, R, ghinv, G);
+      const int ising = call_jac_ghimj<M>(Y.b, F.b, R.b, ghinv, G.b);
       s.Ndec++;
       if (ising == 0) break;
       s.Nsng++;
@@ -253,7 +287,7 @@ struct Cell {
         return -8;
       }
     }
-    call_decomp<M>(G);
+    call_decomp<M>(G.b);
 
     const double H = s.H;
 #pragma unroll
@@ -277,7 +311,7 @@ struct Cell {
         }
         const double Tau = s.T + m.Alpha[istage - 1] * Dir * H;
         rates_at(p, s, Tau);
-        call_fun<M>(Ynew, F, R, Fcn);
+        call_fun<M>(Ynew.b, F.b, R.b, Fcn.b);
         s.Nfun++;
       }
       Acc Fc = from_fcn0 ? Fcn0 : Fcn;
@@ -296,7 +330,7 @@ struct Cell {
         if (use_g) k = k + HG * dFdT[i];
         Ki[i] = k;
       }
-      call_solve<M>(G, Ki);
+      call_solve<M>(G.b, Ki.b);
       s.Nsol++;
     }
     // new solution, error estimate and ros_ErrorNorm (:582-591, :632-659)
@@ -342,11 +376,14 @@ struct Cell {
   }
 };
 
-// Plain kernel: one lane per cell for the whole integration.
+// Plain kernel: one lane per cell for the whole integration.  The first
+// p.lanes lanes of each wavefront carry cells (p.lanes < 64 puts more
+// wavefronts in flight for small batches).
 template <class M, int S>
-__global__ __launch_bounds__(64) void ros_plain_kernel(RosParams p) {
+__global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void ros_plain_kernel(RosParams p) {
   using L = Layout<M, S>;
-  const int k = blockIdx.x * TILE + threadIdx.x;
+  if ((int)threadIdx.x >= p.lanes) return;
+  const int k = blockIdx.x * p.lanes + threadIdx.x;
   if (k >= p.ncells) return;
   Cell<M, S> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + threadIdx.x};
   CellState s;
@@ -361,25 +398,27 @@ __global__ __launch_bounds__(64) void ros_plain_kernel(RosParams p) {
 // one lane claims as many queue entries as there are idle lanes with a
 // single atomic and the idle lanes take consecutive entries.  A wavefront
 // leaves once the queue is exhausted and none of its lanes is active, so
-// every wavefront reaches the exit.
+// every wavefront reaches the exit.  Lanes at or above p.lanes never work.
 template <class M, int S>
-__global__ __launch_bounds__(64) void ros_persistent_kernel(RosParams p) {
+__global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void ros_persistent_kernel(RosParams p) {
   using L = Layout<M, S>;
   const int lane = threadIdx.x;
   Cell<M, S> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + lane};
   CellState s;
+  const bool worker = lane < p.lanes;
   bool active = false;
   bool exhausted = false;  // wavefront-uniform
   for (;;) {
-    const uint64_t idle = __ballot(!active);
+    const uint64_t idle = __ballot(worker && !active);
     const int n_idle = __popcll(idle);
-    if (!exhausted && n_idle > 0 && (n_idle >= p.refill_min || n_idle == TILE)) {
+    const int n_active = __popcll(__ballot(active));
+    if (!exhausted && n_idle > 0 && (n_idle >= p.refill_min || n_active == 0)) {
       const int first = __ffsll((long long)idle) - 1;
       int base = 0;
       if (lane == first) base = atomicAdd(p.queue, n_idle);
       base = __shfl(base, first);
       if (base + n_idle >= p.ncells) exhausted = true;
-      if (!active) {
+      if (worker && !active) {
         const int rank = __popcll(idle & ((1ull << lane) - 1ull));
         const int k = base + rank;
         if (k < p.ncells) {

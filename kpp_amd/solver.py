@@ -20,7 +20,7 @@ from pathlib import Path
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "lib" / "libkpphip.so"
+LIB_PATH = Path(os.environ.get("KPPHIP_LIB", PKG / "lib" / "libkpphip.so"))
 GEN = PKG / "_gen"
 
 SUN_FIXED = 0
@@ -63,7 +63,8 @@ class _Batch(ctypes.Structure):
 
 
 class _Launch(ctypes.Structure):
-    _fields_ = [("variant", ctypes.c_int), ("nblocks", ctypes.c_int), ("refill_min", ctypes.c_int)]
+    _fields_ = [("variant", ctypes.c_int), ("nblocks", ctypes.c_int), ("refill_min", ctypes.c_int),
+                ("lanes", ctypes.c_int)]
 
 
 _lib = None
@@ -201,15 +202,15 @@ _default_ws = {}
 
 
 def workspace_bytes(mech: str, method: int, ncells: int, variant=PLAIN, nblocks=0,
-                    refill_min=0) -> int:
-    l = _Launch(variant, nblocks, refill_min)
+                    refill_min=0, lanes=0) -> int:
+    l = _Launch(variant, nblocks, refill_min, lanes)
     return int(library().kpp_workspace_bytes(mech.encode(), method, ctypes.byref(l), ncells))
 
 
 def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, RPAR=None,
                sun=None, sun_mode=SUN_FIXED, hstart=None, rconst=None, order=None,
                ierr=None, istat=None, rstat=None, variant=PLAIN, nblocks=0, refill_min=0,
-               workspace=None, stream=None):
+               lanes=0, workspace=None, stream=None):
     """Batched Rosenbrock on device tensors (float64 SoA, C is [NSPEC, ncells]).
 
     tstart/tend: per-cell tensors or floats.  atol/rtol: host sequences of
@@ -255,7 +256,7 @@ def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, R
     RPAR = np.zeros(20) if RPAR is None else np.ascontiguousarray(RPAR, np.float64)
     atol = np.ascontiguousarray(np.atleast_1d(atol), np.float64)
     rtol = np.ascontiguousarray(np.atleast_1d(rtol), np.float64)
-    l = _Launch(variant, nblocks, refill_min)
+    l = _Launch(variant, nblocks, refill_min, lanes)
     nbytes = L.kpp_workspace_bytes(mech.encode(), int(IPAR[3]), ctypes.byref(l), n)
     if workspace is None:
         workspace = _default_ws.setdefault(str(dev), Workspace(dev))
