@@ -961,6 +961,148 @@ void emit_c_model(Model& md, const std::string& path) {
   o << "#endif\n";
 }
 
+// ---------------------------------------------------------------------------
+// Memory-pipelined emission of KppDecomp and KppSolve for the device.
+//
+// The routines are rewritten as a list of single-operation statements in the
+// reference's order (one subtraction or division per statement, so the
+// floating-point results are unchanged), over array elements with versions.
+// Each operand is then taken either from a register (its producer is at most
+// `window` statements back) or from memory through a load that is issued
+// `dist` reads ahead of its use, but never before the store of the version it
+// reads.  sched_barrier every `chunk` statements keeps the compiler from
+// sinking the loads back next to their uses, so a wavefront keeps `dist`
+// loads in flight instead of one.
+// ---------------------------------------------------------------------------
+struct PipeParams {
+  int dist = 24, window = 48, chunk = 12;
+};
+PipeParams g_pipe;
+
+struct PipeOpd {
+  int arr, idx, ver;
+};
+struct PipeStmt {
+  int arr, idx;                // element written (new version)
+  std::string expr;            // with $0, $1, ... for operands
+  std::vector<PipeOpd> opds;
+};
+
+struct Pipeline {
+  std::vector<std::string> arr_name;  // accessor names, e.g. "G", "X"
+  std::vector<PipeStmt> st;
+  std::map<std::pair<int, int>, int> cur;               // (arr, idx) -> current version
+  std::map<std::tuple<int, int, int>, int> producer;    // version -> statement
+
+  PipeOpd read(int a, int i) { return {a, i, cur[{a, i}]}; }
+  void assign(int a, int i, const std::string& e, std::vector<PipeOpd> ops) {
+    int t = (int)st.size();
+    st.push_back({a, i, e, ops});
+    int v = ++cur[{a, i}];
+    producer[{a, i, v}] = t;
+  }
+
+  void emit(std::ostream& o, const PipeParams& pp) {
+    int n = (int)st.size();
+    auto vname = [](int a, int i, int v) {
+      return "r" + std::to_string(a) + "_" + std::to_string(i) + "_" + std::to_string(v);
+    };
+    // operand sources
+    struct Read {
+      int use, earliest, arr, idx, ver;
+    };
+    std::vector<Read> reads;
+    std::vector<std::vector<std::string>> opname(n);
+    std::set<std::tuple<int, int, int>> must_store;
+    for (int t = 0; t < n; t++) {
+      for (auto& od : st[t].opds) {
+        auto it = producer.find({od.arr, od.idx, od.ver});
+        int prod = (it == producer.end()) ? -1 : it->second;
+        if (prod >= 0 && t - prod <= pp.window) {
+          opname[t].push_back(vname(od.arr, od.idx, od.ver));
+        } else {
+          opname[t].push_back("m" + std::to_string(reads.size()));
+          reads.push_back({t, prod >= 0 ? prod + 1 : 0, od.arr, od.idx, od.ver});
+          if (prod >= 0) must_store.insert({od.arr, od.idx, od.ver});
+        }
+      }
+    }
+    for (auto& kv : cur)  // final versions are the routine's output
+      if (kv.second > 0) must_store.insert({kv.first.first, kv.first.second, kv.second});
+    // last read used at each statement
+    std::vector<int> last_read_at(n, -1), first_read_from(n + 1, (int)reads.size());
+    for (int r = 0; r < (int)reads.size(); r++) last_read_at[reads[r].use] = r;
+    for (int t = n - 1, r = (int)reads.size(); t >= 0; t--) {
+      while (r > 0 && reads[r - 1].use >= t) r--;
+      first_read_from[t] = r;
+    }
+    int issued = 0;
+    std::map<std::pair<int, int>, int> ver_now;
+    for (int t = 0; t < n; t++) {
+      int target = std::min((int)reads.size(), first_read_from[t] + pp.dist);
+      target = std::max(target, last_read_at[t] + 1);
+      while (issued < target && reads[issued].earliest <= t) {
+        auto& r = reads[issued];
+        o << "  const double m" << issued << " = " << arr_name[r.arr] << "[" << r.idx << "];\n";
+        issued++;
+      }
+      if (issued <= last_read_at[t]) die("pipeline: operand not loadable in time");
+      auto& s = st[t];
+      std::string e = s.expr;
+      for (int k = (int)s.opds.size() - 1; k >= 0; k--) {
+        std::string key = "$" + std::to_string(k);
+        size_t p;
+        while ((p = e.find(key)) != std::string::npos) e.replace(p, key.size(), opname[t][k]);
+      }
+      int v = ++ver_now[{s.arr, s.idx}];
+      o << "  const double " << vname(s.arr, s.idx, v) << " = " << e << ";\n";
+      if (must_store.count({s.arr, s.idx, v}))
+        o << "  " << arr_name[s.arr] << "[" << s.idx << "] = " << vname(s.arr, s.idx, v) << ";\n";
+      if ((t + 1) % pp.chunk == 0) o << "  __builtin_amdgcn_sched_barrier(0);\n";
+    }
+  }
+};
+
+void emit_pipelined(std::ostream& o, const Structure& S, const PipeParams& pp) {
+  const int G = 0, X = 1;
+  {  // KppDecomp (util/sutil.c:2-23), row-wise elimination in place
+    Pipeline pl;
+    pl.arr_name = {"G"};
+    for (int k = 0; k < S.nvar; k++) {
+      std::map<int, int> pos;
+      for (int kk = S.lu_crow[k]; kk < S.lu_crow[k + 1]; kk++) pos[S.lu_icol[kk]] = kk;
+      for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
+        int j = S.lu_icol[kk];
+        pl.assign(G, kk, "$0/$1", {pl.read(G, kk), pl.read(G, S.lu_diag[j])});
+        for (int jj = S.lu_diag[j] + 1; jj < S.lu_crow[j + 1]; jj++) {
+          int p = pos.at(S.lu_icol[jj]);
+          pl.assign(G, p, "$0 - $1*$2", {pl.read(G, p), pl.read(G, kk), pl.read(G, jj)});
+        }
+      }
+    }
+    o << "// KppDecomp with memory pipelining (same operations, same order)\n";
+    o << "template <class TG>\nstatic __device__ __forceinline__ void KppDecompPF(TG G) {\n";
+    pl.emit(o, pp);
+    o << "}\n";
+  }
+  {  // KppSolve (src/gen.c:1960-2033), one subtraction per statement
+    Pipeline pl;
+    pl.arr_name = {"G", "X"};
+    for (int i = 0; i < S.nvar; i++)
+      for (int j = S.lu_crow[i]; j < S.lu_diag[i]; j++)
+        pl.assign(X, i, "$0 - $1*$2", {pl.read(X, i), pl.read(G, j), pl.read(X, S.lu_icol[j])});
+    for (int i = S.nvar - 1; i >= 0; i--) {
+      for (int j = S.lu_diag[i] + 1; j < S.lu_crow[i + 1]; j++)
+        pl.assign(X, i, "$0 - $1*$2", {pl.read(X, i), pl.read(G, j), pl.read(X, S.lu_icol[j])});
+      pl.assign(X, i, "$0/$1", {pl.read(X, i), pl.read(G, S.lu_diag[i])});
+    }
+    o << "// KppSolve with memory pipelining (same operations, same order)\n";
+    o << "template <class TG, class TX>\nstatic __device__ __forceinline__ void KppSolvePF(TG G, TX X) {\n";
+    pl.emit(o, pp);
+    o << "}\n";
+  }
+}
+
 // The HIP header: everything is a template over an element accessor, so the
 // same straight-line code runs on SoA global memory (p[i*stride]), LDS or
 // register arrays.  All indices are compile-time constants.
@@ -1098,6 +1240,7 @@ void emit_hip(Model& md, const std::string& path) {
     if (checked) o << "  return 0;\n";
     o << "}\n";
   }
+  emit_pipelined(o, S, g_pipe);
   o << "template <class TG, class TX>\nstatic __device__ __forceinline__ void KppSolve(TG JVS, "
        "TX X) {\n";
   for (int i = 0; i < S.nvar; i++) {
@@ -1173,6 +1316,9 @@ int main(int argc, char** argv) {
     else if (a == "--root") root = next();
     else if (a == "--kpp-home") kpp_home = next();
     else if (a == "--no-reorder") do_reorder = false;
+    else if (a == "--pf-dist") g_pipe.dist = atoi(next().c_str());
+    else if (a == "--pf-window") g_pipe.window = atoi(next().c_str());
+    else if (a == "--pf-chunk") g_pipe.chunk = atoi(next().c_str());
     else if (a == "-h" || a == "--help") {
       printf("usage: kppgen [--kpp-home DIR] [--root NAME] [-o OUTDIR] mechanism.{def,kpp}\n");
       return 0;

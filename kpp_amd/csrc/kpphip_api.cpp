@@ -159,7 +159,7 @@ struct Geometry {
   int nblocks;
   int refill_min;
   int lanes;
-  size_t tol_bytes, tile_bytes, total;
+  size_t tol_bytes, tile_bytes, rpre_bytes, total;
 };
 
 bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry& g) {
@@ -193,7 +193,10 @@ bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry
   }
   g.tol_bytes = align256(2 * (size_t)m->nvar * sizeof(double));
   g.tile_bytes = (size_t)rows * 64 * sizeof(double);
-  g.total = kHead + g.tol_bytes + (size_t)g.nblocks * g.tile_bytes;
+  // persistent kernel: rate constants of every cell from a pre-pass
+  g.rpre_bytes = (variant == KPP_VARIANT_PERSISTENT)
+                     ? align256((size_t)m->nreact * (size_t)ncells * sizeof(double)) : 0;
+  g.total = kHead + g.tol_bytes + g.rpre_bytes + (size_t)g.nblocks * g.tile_bytes;
   return true;
 }
 
@@ -336,7 +339,8 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
   p.rstat = b->rstat;
   p.order = b->order;
   p.queue = (int*)ws;
-  p.scratch = (double*)(ws + kHead + g.tol_bytes);
+  p.rpre = (g.rpre_bytes && !p.rates_given) ? (double*)(ws + kHead + g.tol_bytes) : nullptr;
+  p.scratch = (double*)(ws + kHead + g.tol_bytes + g.rpre_bytes);
   p.refill_min = g.refill_min;
   p.lanes = g.lanes;
   const int variant = l ? l->variant : KPP_VARIANT_PLAIN;

@@ -39,6 +39,7 @@ struct RosParams {
   double Roundoff, Hmin, Hmax_user, Hstart_user, FacMin, FacMax, FacRej, FacSafe;
   const double* hstart;    // [ncells] per-cell starting step or null
   const double* rconst;    // [NREACT][ldc] given rate constants or null
+  double* rpre;            // [NREACT][ncells] rate constants computed by a pre-pass, or null
   int rates_given;
   int dfdt_zero;           // rates do not depend on T: dF/dT is exactly zero
   int stat0[8];            // IPAR[10..17] on entry

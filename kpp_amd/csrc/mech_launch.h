@@ -68,6 +68,8 @@ template <class M, int S>
 hipError_t launch_s(const RosParams& p, int variant, int nblocks, hipStream_t st) {
   if (variant == KPP_VARIANT_PERSISTENT) {
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, p.queue);
+    if (p.rpre)
+      hipLaunchKernelGGL(ros_rates_prepass<M>, dim3((p.ncells + 255) / 256), dim3(256), 0, st, p);
     hipLaunchKernelGGL((ros_persistent_kernel<M, S>), dim3(nblocks), dim3(TILE), 0, st, p);
   } else {
     hipLaunchKernelGGL((ros_plain_kernel<M, S>), dim3(nblocks), dim3(TILE), 0, st, p);

@@ -39,6 +39,13 @@
 #ifndef KPPHIP_MODEL_INLINE
 #define KPPHIP_MODEL_INLINE 0
 #endif
+//  KPPHIP_PIPELINED     1: the integrator uses the memory-pipelined
+//                       KppDecompPF/KppSolvePF emitted by the compiler.
+//                       Off: with it, a persistent-kernel parity run stopped
+//                       making progress on the GPU for a reason not yet found.
+#ifndef KPPHIP_PIPELINED
+#define KPPHIP_PIPELINED 0
+#endif
 #if KPPHIP_WAVES_PER_EU > 0
 #define KPPHIP_KERNEL_ATTR __attribute__((amdgpu_waves_per_eu(KPPHIP_WAVES_PER_EU)))
 #else
@@ -125,11 +132,19 @@ KPPHIP_MODEL_CALL int call_jac_ghimj(dptr V, dptr F, dptr R, double ghinv, dptr 
 }
 template <class M>
 KPPHIP_MODEL_CALL void call_decomp(dptr G) {
+#if KPPHIP_PIPELINED
+  M::KppDecompPF(Acc{G});
+#else
   M::KppDecomp(Acc{G});
+#endif
 }
 template <class M>
 KPPHIP_MODEL_CALL void call_solve(dptr G, dptr X) {
+#if KPPHIP_PIPELINED
+  M::KppSolvePF(Acc{G}, Acc{X});
+#else
   M::KppSolve(Acc{G}, Acc{X});
+#endif
 }
 template <class M>
 KPPHIP_MODEL_CALL void call_rates_sun(double TEMP, double CFACTOR, double SUN, double TIME,
@@ -184,6 +199,10 @@ struct Cell {
     for (int i = 0; i < M::NFIX; i++) F[i] = p.C[(M::NVAR + i) * ld + cell];
     if (p.rates_given) {
       for (int i = 0; i < M::NREACT; i++) R[i] = p.rconst[i * ld + cell];
+    } else if (p.rpre) {
+      // computed by ros_rates_prepass with the same inputs and formulae
+      const size_t n = (size_t)p.ncells;
+      for (int i = 0; i < M::NREACT; i++) R[i] = p.rpre[i * n + cell];
     } else {
       call_rates_fixed<M>(s.TEMP, s.CFACTOR, R.b);
       if (p.sun_mode == KPP_SUN_FIXED && !M::RATES_USE_TIME)
@@ -375,6 +394,26 @@ struct Cell {
     return 0;
   }
 };
+
+// Rate constants a cell starts with (what Cell::start computes), for all
+// cells at once, so that a persistent-kernel lane refilled with a new cell
+// copies them instead of evaluating exp/pow while its wavefront waits.
+template <class M>
+__global__ __launch_bounds__(256) void ros_rates_prepass(RosParams p) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= p.ncells) return;
+  double* base = p.rpre + cell;
+  const size_t n = (size_t)p.ncells;
+  struct S {
+    double* b;
+    size_t n;
+    __device__ __forceinline__ double& operator[](int i) const { return b[(size_t)i * n]; }
+  } r{base, n};
+  const double TEMP = p.temp[cell], CF = p.cfactor[cell];
+  M::rates_fixed(TEMP, CF, r);
+  if (p.sun_mode == KPP_SUN_FIXED && !M::RATES_USE_TIME)
+    M::rates_sun(TEMP, CF, p.sun ? p.sun[cell] : 0.0, 0.0, r);
+}
 
 // Plain kernel: one lane per cell for the whole integration.  The first
 // p.lanes lanes of each wavefront carry cells (p.lanes < 64 puts more
