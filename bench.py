@@ -29,6 +29,11 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+HERE = os.path.dirname(os.path.abspath(__file__))
+# HBM bytes per launch of the tuned kernel on the default workload, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by the same counters read on
+# a copy of known size (``--copy-probe``); the file records the commands.
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_ros_persistent_saprc99_uneven_1M.json")
 
 
 def log(*a):
@@ -80,6 +85,29 @@ def cpu_baseline(wl_fn, ncells_full, seed, target_s=12.0):
                       f"{dt:.1f} s", "ierr_ok": bool((out["ierr"] == 1).all())}
 
 
+def copy_probe(nbytes=1 << 30, reps=3):
+    """A device-to-device copy of known size, for calibrating HBM counters."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
+                      "reps": reps}))
+
+
+def measured_traffic(workload, ncells):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or t.get("ncells") != ncells:
+        return None
+    return t.get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,12 +120,17 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--nblocks", type=int, default=0)
     ap.add_argument("--refill-min", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): cells are "
               "independent and shard without communication; run one process per GPU.",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import torch
 
@@ -183,7 +216,8 @@ def main():
                    "path": "persistent work-stealing kernel", "all_ierr_1": ok_t,
                    "attempts_per_cell": attempts / n},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, n),
                      "kernel": "ros_persistent_kernel<kppm_saprc99,4>",
                      "kernel_ms": kern_t * 1e3, "algorithmic_bytes": nbytes,
                      "fp64_tflops": nflops / kern_t / 1e12,
