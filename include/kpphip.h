@@ -79,6 +79,8 @@ typedef struct {
   int* istat;            /* device [8][ld] IPAR[10..17] per cell, or NULL */
   double* rstat;         /* device [2][ld] Texit, Hexit per cell, or NULL */
   const int* order;      /* device [ncells] processing order (a permutation) or NULL */
+  double* hfirst;        /* device [ncells] step log: size of each cell's first accepted step
+                            (0 if none was accepted), or NULL.  Not a reference output. */
 } kpp_batch;
 
 typedef struct {
@@ -141,6 +143,55 @@ int kpp_get_state(kpp_ctx* ctx, double* C, int* ierr, int* istat, double* rstat)
  * cell carried from the previous call's Hexit. */
 int kpp_ctx_set_tol(kpp_ctx* ctx, const double* atol, const double* rtol, int vector_tol);
 int INTEGRATE_BATCH(kpp_ctx* ctx, double TIN, double TOUT);
+
+/* ---- learned step scheduler (no counterpart in KPP) ----
+ * A 4-layer MLP (in_pad -> 128 -> 128 -> 128 -> 16, GELU) on BF16 matrix
+ * cores with FP32 accumulation predicts, from a cell's initial state, the
+ * size of its first accepted step (output 0, standardised log10) and one of
+ * 8 step-count buckets (outputs 1..8, logits).  The prediction only sets the
+ * per-cell starting step (hstart, Rosenbrock's RPAR[2]) and the launch order
+ * (order); Rosenbrock's error control still judges every step.
+ * Parameters: FP32 master copy `w` and BF16 copy `wb`, both laid out as
+ * W1[128][in_pad] b1[128] W2[128][128] b2[128] W3[128][128] b3[128]
+ * W4[16][128] b4[16] (kpp_mlp_param_offsets). */
+#define KPP_MLP_HIDDEN 128
+#define KPP_MLP_OUT 16
+#define KPP_MLP_BUCKETS 8
+typedef struct {
+  int nin;    /* inputs used: NSPEC + 2 (log10 concentrations, TEMP, SUN) */
+  int in_pad; /* padded input width, a multiple of 32 */
+  float* w;   /* device FP32 parameters, kpp_mlp_param_count(in_pad) floats */
+  void* wb;   /* device BF16 parameters, same layout */
+} kpp_mlp;
+size_t kpp_mlp_param_count(int in_pad);
+int kpp_mlp_param_offsets(int in_pad, size_t w_off[4], size_t b_off[4]);
+/* refresh wb from w */
+int kpp_mlp_sync_bf16(const kpp_mlp* net, void* stream);
+/* features X[n][in_pad] (BF16): (log10(max(C,cfloor)) - mu)/sd per species,
+ * then TEMP and SUN (0 if sun is NULL) standardised the same way, then zeros */
+int kpp_mlp_features(int n, int ld, int nspec, const double* C, const double* temp,
+                     const double* sun, const float* mu, const float* sd, double cfloor,
+                     int in_pad, void* X, void* stream);
+size_t kpp_mlp_work_bytes(int n);
+/* out[n][16] = network outputs (FP32) */
+int kpp_mlp_forward(const kpp_mlp* net, const void* X, int n, float* out, void* work,
+                    size_t work_bytes, void* stream);
+/* gradients of the mean loss  w*(o0-y)^2 + cross-entropy(o[1..8], bucket)
+ * in the parameter layout (FP32); *loss (device) = summed loss */
+int kpp_mlp_grad(const kpp_mlp* net, const void* X, const float* y, const int* bucket,
+                 const float* wreg, int n, float* grad, float* loss, void* work,
+                 size_t work_bytes, void* stream);
+/* kpp_mlp_grad followed by one Adam step (beta 0.9/0.999, eps 1e-8) on w, wb */
+int kpp_mlp_train_step(const kpp_mlp* net, const void* X, const float* y, const int* bucket,
+                       const float* wreg, int n, float* grad, float* m1, float* m2, int step,
+                       float lr, float* loss, void* work, size_t work_bytes, void* stream);
+/* prediction: hstart[n] = 10^(o0*sd_y+mu_y) clamped to [hmin,hmax], bucket[n],
+ * order[n] = cells sorted by predicted bucket, most steps first (stable);
+ * any output may be NULL */
+size_t kpp_mlp_schedule_work_bytes(int n);
+int kpp_mlp_schedule(const kpp_mlp* net, const void* X, int n, float mu_y, float sd_y,
+                     double hmin, double hmax, double* hstart, int* bucket, int* order,
+                     void* work, size_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -100,7 +100,8 @@ def _hipcc_obj(src: Path, out: Path):
 def build_lib(mechs: list[str], jobs: int = 8) -> Path:
     OBJ.mkdir(parents=True, exist_ok=True)
     LIB.mkdir(parents=True, exist_ok=True)
-    srcs = [GEN / f"kpphip_{m}.hip" for m in mechs] + [CSRC / "kpphip_api.cpp"]
+    srcs = [GEN / f"kpphip_{m}.hip" for m in mechs] + [CSRC / "kpphip_api.cpp",
+                                                        CSRC / "mlp.hip"]
     objs = [OBJ / (s.stem + ".o") for s in srcs]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_hipcc_obj, srcs, objs))

@@ -337,6 +337,7 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
   p.ierr = b->ierr;
   p.istat = b->istat;
   p.rstat = b->rstat;
+  p.hfirst = b->hfirst;
   p.order = b->order;
   p.queue = (int*)ws;
   p.rpre = (g.rpre_bytes && !p.rates_given) ? (double*)(ws + kHead + g.tol_bytes) : nullptr;

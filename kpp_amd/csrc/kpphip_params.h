@@ -46,6 +46,7 @@ struct RosParams {
   int* ierr;               // [ncells]
   int* istat;              // [8][ldc] or null
   double* rstat;           // [2][ldc] (Texit, Hexit) or null
+  double* hfirst;          // [ncells] size of each cell's first accepted step, or null
   const int* order;        // [ncells] launch order (cell permutation) or null
   double* scratch;         // tiles
   int* queue;              // persistent kernel: next-cell counter (starts at 0)

@@ -375,6 +375,8 @@ struct Cell {
     double Hnew = H * Fac;
     s.Nstp++;
     if ((Err <= 1.0) || (H <= p.Hmin)) {
+      // step log: size of the first accepted step of this call
+      if (p.hfirst && s.Nacc == p.stat0[3]) p.hfirst[s.cell] = H;
       s.Nacc++;
       for (int i = 0; i < NVAR; i++) Y[i] = Ynew[i];
       s.T += Dir * H;

@@ -58,7 +58,7 @@ class _Batch(ctypes.Structure):
         ("tstart0", ctypes.c_double), ("tend0", ctypes.c_double),
         ("hstart", ctypes.c_void_p), ("rconst", ctypes.c_void_p),
         ("ierr", ctypes.c_void_p), ("istat", ctypes.c_void_p), ("rstat", ctypes.c_void_p),
-        ("order", ctypes.c_void_p),
+        ("order", ctypes.c_void_p), ("hfirst", ctypes.c_void_p),
     ]
 
 
@@ -209,14 +209,15 @@ def workspace_bytes(mech: str, method: int, ncells: int, variant=PLAIN, nblocks=
 
 def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, RPAR=None,
                sun=None, sun_mode=SUN_FIXED, hstart=None, rconst=None, order=None,
-               ierr=None, istat=None, rstat=None, variant=PLAIN, nblocks=0, refill_min=0,
+               hfirst=None, ierr=None, istat=None, rstat=None, variant=PLAIN, nblocks=0, refill_min=0,
                lanes=0, workspace=None, stream=None):
     """Batched Rosenbrock on device tensors (float64 SoA, C is [NSPEC, ncells]).
 
     tstart/tend: per-cell tensors or floats.  atol/rtol: host sequences of
     NVAR entries (IPAR[1]==0) or 1.  Returns (code, ierr, istat, rstat) with
     code 0 or Rosenbrock's parameter error; launches asynchronously on
-    ``stream`` (torch's current stream by default).
+    ``stream`` (torch's current stream by default).  ``hfirst`` (float64
+    [ncells], optional) receives each cell's first accepted step size.
     """
     import torch
     L = library()
@@ -252,6 +253,7 @@ def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, R
     b.istat = istat.data_ptr()
     b.rstat = rstat.data_ptr()
     b.order = order.data_ptr() if order is not None else None
+    b.hfirst = hfirst.data_ptr() if hfirst is not None else None
     IPAR = np.ascontiguousarray(IPAR, np.int32)
     RPAR = np.zeros(20) if RPAR is None else np.ascontiguousarray(RPAR, np.float64)
     atol = np.ascontiguousarray(np.atleast_1d(atol), np.float64)

@@ -26,7 +26,7 @@ def lib(mech: str, fast: bool = False) -> ctypes.CDLL:
     L = ctypes.CDLL(str(p))
     vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     L.ros_ref_batch.argtypes = [ci, vp, vp, vp, vp, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
-                                vp, ci]
+                                vp, vp, ci]
     L.ros_ref_update_sun.restype = cd
     L.ros_ref_update_sun.argtypes = [cd]
     L.ros_ref_wlamch.restype = cd
@@ -68,6 +68,7 @@ def integrate(batch, nthreads: int = 0, fast: bool = False, hstart=None, rconst=
     ierr = np.zeros(n, np.int32)
     istat = np.zeros((8, n), np.int32)
     rstat = np.zeros((2, n), np.float64)
+    hacc1 = np.zeros(n, np.float64)
     f = lambda a: np.ascontiguousarray(a, np.float64)
     ip = np.ascontiguousarray(batch.IPAR if IPAR is None else IPAR, np.int32)
     rp = f(batch.RPAR if RPAR is None else RPAR)
@@ -78,8 +79,9 @@ def integrate(batch, nthreads: int = 0, fast: bool = False, hstart=None, rconst=
     L.ros_ref_batch(n, _p(Cw), _p(temp), _p(cf), _p(sun),
                     batch.sun_mode if sun_mode is None else sun_mode, _p(ts), _p(te), _p(atol),
                     _p(rtol), _p(ip), _p(rp), _p(hs), _p(rc), _p(ierr), _p(istat), _p(rstat),
-                    nthreads)
-    return {"C": Cw, "ierr": ierr, "istat": istat, "rstat": rstat, "nvar": sz["NVAR"]}
+                    _p(hacc1), nthreads)
+    return {"C": Cw, "ierr": ierr, "istat": istat, "rstat": rstat, "hacc1": hacc1,
+            "nvar": sz["NVAR"]}
 
 
 def method(code: int):

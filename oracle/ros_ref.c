@@ -64,6 +64,7 @@ typedef struct {
   double RCONST[KPP_NREACT];
   int rates_given;
   ros_stats st;
+  double Hacc1; /* step size of the first accepted step of the call (step log), 0 if none */
 } ros_cell;
 
 typedef struct {
@@ -428,6 +429,7 @@ static int RosenbrockIntegrator(ros_cell* c, double* Y, double Tstart, double Te
       Hnew = H * Fac;
       c->st.Nstp++;
       if ((Err <= ONE) || (H <= Hmin)) {
+        if (c->Hacc1 == ZERO) c->Hacc1 = H;
         c->st.Nacc++;
         WCOPY(KPP_NVAR, Ynew, Y);
         T += Direction * H;
@@ -564,12 +566,14 @@ double ros_ref_error_norm(const double* Y, const double* Ynew, const double* Yer
  * hstart: per-cell starting step (overrides RPAR[2]) or NULL.
  * rconst: per-cell rate constants (SoA NREACT x ncells) or NULL.
  * ierr[c], istat[8*ncells] (Nfun..Nsng, SoA), rstat[2*ncells] (Texit, Hexit).
+ * hacc1[c]: step size of the cell's first accepted step, or NULL (a step log
+ * for the learned scheduler; not part of the reference's outputs).
  */
 void ros_ref_batch(int ncells, double* C, const double* temp, const double* cfactor,
                    const double* sun, int sun_mode, const double* tstart, const double* tend,
                    const double* atol, const double* rtol, const int* IPAR_in,
                    const double* RPAR_in, const double* hstart, const double* rconst, int* ierr,
-                   int* istat, double* rstat, int nthreads) {
+                   int* istat, double* rstat, double* hacc1, int nthreads) {
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 16)
@@ -601,6 +605,7 @@ void ros_ref_batch(int ncells, double* C, const double* temp, const double* cfac
       rstat[cidx] = RPAR[10];
       rstat[(size_t)ncells + cidx] = RPAR[11];
     }
+    if (hacc1) hacc1[cidx] = c.Hacc1;
   }
 }
 
