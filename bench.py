@@ -3,11 +3,21 @@
 One step = one batched integration of the whole synthetic batch (default:
 1,048,576 saprc99 cells, 50 % night / 50 % day with SUN ~ U(0.2, 1),
 Rodas3, FP64), inputs resident in HBM; the concentrations are reset from a
-pristine device copy at the start of each step.  ``value`` is the tuned path
-(persistent work-stealing kernel); the plain one-cell-per-lane kernel is
-timed on the same cells and reported beside it, and the CPU oracle (this
-project's C restatement of KPP's int/rosenbrock.c, OpenMP over cells) is
-timed on a bounded sample of the same cells on the host's cores.
+pristine device copy at the start of each step.  ``value`` is the tuned path:
+the learned scheduler (network inference on the batch: per-cell starting
+step and a launch order by predicted step count) followed by the persistent
+work-stealing kernel, all inside the timed region.  The plain
+one-cell-per-lane kernel (Rosenbrock's default start, cells in index order)
+and the persistent kernel without the scheduler are timed on the same cells
+and reported beside it, and the CPU oracle (this project's C restatement of
+KPP's int/rosenbrock.c, OpenMP over cells) is timed on a bounded sample of
+the same cells on the host's cores.
+
+Accuracy (untimed): on an evenly spaced sample of cells, the final states of
+the plain and tuned runs are compared with an FP64 integration of the same
+cells at RTOL 1e-10 (ATOL 1e-3) by the plain kernel; max / median / 99th
+percentile relative errors are reported for species above ATOL and above
+ATOL/RTOL.
 
 Roofline: the solver is bound by HBM bandwidth.  Algorithmic bytes per step
 attempt (one unit) follow the streamed working set of each attempt:
@@ -34,6 +44,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by the same counters read on
 # a copy of known size (``--copy-probe``); the file records the commands.
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_ros_persistent_saprc99_uneven_1M.json")
+WEIGHTS = os.path.join(HERE, "kpp_amd", "weights", "saprc99_step_mlp.npz")
 
 
 def log(*a):
@@ -64,7 +75,7 @@ def algorithmic_work(mech, method, istat):
             first + retry, {"bytes_first_attempt": b_first, "bytes_retry": b_retry})
 
 
-def cpu_baseline(wl_fn, ncells_full, seed, target_s=12.0):
+def cpu_baseline(wl_fn, ncells_full, seed, target_s=20.0):
     """Time the oracle on a bounded sample of the same workload."""
     from oracle import ref
     threads = len(os.sched_getaffinity(0))
@@ -108,6 +119,17 @@ def measured_traffic(workload, ncells):
     return t.get("bytes_per_launch")
 
 
+def rel_errors(X, ref, nvar, atol, rtol):
+    """max / median / p99 relative error of X against ref for species above thresholds."""
+    out = {}
+    for name, thr in (("above_atol", atol), ("above_atol_over_rtol", atol / rtol)):
+        m = np.abs(ref[:nvar]) > thr
+        r = np.abs(X[:nvar] - ref[:nvar])[m] / np.abs(ref[:nvar])[m]
+        out[name] = {"max": float(r.max()), "median": float(np.median(r)),
+                     "p99": float(np.percentile(r, 99)), "n": int(m.sum())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +142,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--nblocks", type=int, default=0)
     ap.add_argument("--refill-min", type=int, default=0)
+    ap.add_argument("--weights", default=WEIGHTS, help="trained step-scheduler network")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--accuracy-cells", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -150,53 +175,109 @@ def main():
     istat = torch.empty((8, n), dtype=torch.int32, device=dev)
     rstat = torch.empty((2, n), dtype=torch.float64, device=dev)
     ws = solver.Workspace(dev)
+    span = float(np.max(np.abs(b.tend - b.tstart)))
 
-    def step(variant):
+    model = None
+    if not args.no_learned:
+        from kpp_amd import learned
+        if os.path.exists(args.weights):
+            model = learned.StepModel.load(args.weights, dev)
+            log(f"[bench] step scheduler weights: {os.path.relpath(args.weights, HERE)}")
+        else:
+            log("[bench] no trained step scheduler found: training one on step logs of two "
+                "fresh batches (seeds 101, 102; untimed)")
+            model, _ = learned.train_model(b.mech, wl_fn, [101, 102], args.ncells, dev, log=log)
+
+    def step(variant, learned_path):
         C.copy_(C0)
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0, evk, ev1 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         ev0.record()
+        hs = order = None
+        if learned_path:
+            X = model.features(C, d["temp"], d["sun"])
+            hs, _, order = model.schedule(X, 1e-6, span)
+        evk.record()
         code, _, _, _ = solver.rosenbrock(
             b.mech, C, d["temp"], d["cfactor"], tstart=d["tstart"], tend=d["tend"],
             atol=b.atol, rtol=b.rtol, IPAR=b.IPAR, RPAR=b.RPAR, sun=d["sun"],
-            sun_mode=b.sun_mode, ierr=ierr, istat=istat, rstat=rstat, variant=variant,
-            nblocks=args.nblocks if variant == solver.PERSISTENT else 0,
+            sun_mode=b.sun_mode, hstart=hs, order=order, ierr=ierr, istat=istat, rstat=rstat,
+            variant=variant, nblocks=args.nblocks if variant == solver.PERSISTENT else 0,
             refill_min=args.refill_min, workspace=ws)
         ev1.record()
         assert code == 0, code
-        return ev0, ev1
+        return ev0, evk, ev1
 
-    def timed(variant, steps, warmup, name):
+    def timed(variant, learned_path, steps, warmup, name):
         for _ in range(warmup):
-            step(variant)
+            step(variant, learned_path)
         torch.cuda.synchronize()
         evs = []
         t = time.perf_counter()
         for k in range(steps):
-            evs.append(step(variant))
+            evs.append(step(variant, learned_path))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
-        kern = [a.elapsed_time(e) / 1e3 for a, e in evs]
+        dev_t = [a.elapsed_time(e) / 1e3 for a, _, e in evs]
+        kern = [k.elapsed_time(e) / 1e3 for _, k, e in evs]
+        sched = [a.elapsed_time(k) / 1e3 for a, k, _ in evs]
         ok = bool((ierr == 1).all().item())
+        ist = istat.cpu().numpy()
         log(f"[bench] {name}: {steps} steps, {wall / steps * 1e3:.1f} ms/step, "
-            f"kernel {np.mean(kern) * 1e3:.1f} ms, all IERR=1: {ok}")
-        return wall, float(np.mean(kern)), ok
+            f"device {np.mean(dev_t) * 1e3:.1f} ms (scheduler {np.mean(sched) * 1e3:.2f} ms, "
+            f"integrator {np.mean(kern) * 1e3:.1f} ms), attempts/cell {ist[2].mean():.2f}, "
+            f"all IERR=1: {ok}")
+        return {"wall": wall / steps, "device_s": float(np.mean(dev_t)),
+                "kernel_s": float(np.mean(kern)), "sched_s": float(np.mean(sched)), "ok": ok,
+                "istat": ist, "C": C.cpu().numpy()}
 
-    wall_t, kern_t, ok_t = timed(solver.PERSISTENT, args.steps, args.warmup, "tuned (persistent)")
-    ist = istat.cpu().numpy()
-    nbytes, nflops, attempts, bterms = algorithmic_work(mech, b.method, ist)
-    plain = None
+    runs = {}
+    if model is not None:
+        runs["tuned"] = timed(solver.PERSISTENT, True, args.steps, args.warmup,
+                              "tuned (learned schedule + persistent)")
+    runs["persistent"] = timed(solver.PERSISTENT, False, max(1, args.steps if model is None else 1),
+                               args.warmup, "persistent, default start")
     if not args.no_plain:
-        wall_p, kern_p, ok_p = timed(solver.PLAIN, max(1, args.steps), max(0, args.warmup), "plain")
-        plain = {"value": n / (wall_p / max(1, args.steps)), "kernel_ms": kern_p * 1e3,
-                 "all_ierr_1": ok_p}
+        runs["plain"] = timed(solver.PLAIN, False, max(1, args.steps), max(0, args.warmup), "plain")
+    top = runs["tuned"] if model is not None else runs["persistent"]
+
+    # roofline of the persistent kernel on the tuned run's attempts
+    nbytes, nflops, attempts, bterms = algorithmic_work(mech, b.method, top["istat"])
+    kern_t = top["kernel_s"]
+
+    accuracy = None
+    if args.accuracy_cells > 0:
+        import dataclasses
+        cells = np.linspace(0, n - 1, min(n, args.accuracy_cells)).astype(np.int64)
+        tb = wl_fn(seed=args.seed, cells=cells)
+        tb = dataclasses.replace(tb, atol=np.full_like(tb.atol, 1e-3),
+                                 rtol=np.full_like(tb.rtol, 1e-10))
+        td = tb.to_device(dev)
+        t = time.time()
+        code, terr, _, _ = solver.rosenbrock(
+            tb.mech, td["C"], td["temp"], td["cfactor"], tstart=td["tstart"], tend=td["tend"],
+            atol=tb.atol, rtol=tb.rtol, IPAR=tb.IPAR, RPAR=tb.RPAR, sun=td["sun"],
+            sun_mode=tb.sun_mode, variant=solver.PERSISTENT)
+        torch.cuda.synchronize()
+        ref_C = td["C"].cpu().numpy()
+        log(f"[bench] accuracy reference (RTOL 1e-10, ATOL 1e-3, {len(cells)} cells) in "
+            f"{time.time() - t:.1f}s, all IERR=1: {bool((terr == 1).all().item())}")
+        at, rt = float(b.atol[0]), float(b.rtol[0])
+        accuracy = {"reference": f"FP64 RTOL 1e-10 ATOL 1e-3 integration of {len(cells)} "
+                                 f"evenly spaced cells", "sample_cells": int(len(cells))}
+        for k, r in runs.items():
+            accuracy[k] = rel_errors(r["C"][:, cells], ref_C, mech.nvar, at, rt)
+        if "plain" in runs and "tuned" in runs:
+            accuracy["tuned_vs_plain"] = rel_errors(runs["tuned"]["C"][:, cells],
+                                                    runs["plain"]["C"][:, cells], mech.nvar, at, rt)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the CPU oracle on a sample ...")
         cpu = cpu_baseline(wl_fn, n, args.seed)
         log(f"[bench] cpu baseline {cpu['value']:.0f} cells/s on {cpu['cores']} threads")
-    value = n / (wall_t / args.steps)
+    value = n / top["wall"]
     achieved = nbytes / kern_t / 1e9
+    plain = runs.get("plain")
     line = {
         "metric": "cells/s integrated 1 h (saprc99, RTOL 1e-3): tuned vs plain GPU vs CPU",
         "value": value,
@@ -204,7 +285,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall_t / args.steps * 1e3,
+        "ms_per_step": top["wall"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -213,8 +294,10 @@ def main():
         "config": {"workload": args.workload, "mechanism": b.mech, "ncells": n,
                    "method": "Rodas3" if b.method == solver.RODAS3 else int(b.method),
                    "rtol": float(b.rtol[0]), "atol": float(b.atol[0]), "integrate_s": 3600,
-                   "path": "persistent work-stealing kernel", "all_ierr_1": ok_t,
-                   "attempts_per_cell": attempts / n},
+                   "path": ("learned step scheduler (BF16 MFMA network: per-cell first step, "
+                            "launch order by predicted step count) + persistent work-stealing "
+                            "kernel" if model is not None else "persistent work-stealing kernel"),
+                   "all_ierr_1": top["ok"], "attempts_per_cell": attempts / n},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, n),
@@ -222,8 +305,14 @@ def main():
                      "kernel_ms": kern_t * 1e3, "algorithmic_bytes": nbytes,
                      "fp64_tflops": nflops / kern_t / 1e12,
                      "fp64_frac": nflops / kern_t / 1e12 / FP64_PEAK_TFLOPS, **bterms},
-        "plain": plain,
-        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "scheduler_ms": top["sched_s"] * 1e3,
+        "plain": ({"value": n / plain["wall"], "kernel_ms": plain["kernel_s"] * 1e3,
+                   "attempts_per_cell": float(plain["istat"][2].mean()),
+                   "all_ierr_1": plain["ok"]} if plain else None),
+        "persistent_default_start": {"value": n / runs["persistent"]["wall"],
+                                     "attempts_per_cell": float(runs["persistent"]["istat"][2].mean())},
+        "speedup_vs_plain": (value / (n / plain["wall"])) if plain else None,
+        "accuracy": accuracy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
