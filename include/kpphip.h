@@ -144,6 +144,33 @@ int kpp_get_state(kpp_ctx* ctx, double* C, int* ierr, int* istat, double* rstat)
 int kpp_ctx_set_tol(kpp_ctx* ctx, const double* atol, const double* rtol, int vector_tol);
 int INTEGRATE_BATCH(kpp_ctx* ctx, double TIN, double TOUT);
 
+/* ---- autotuner (no counterpart in KPP) ----
+ * Times candidate launch shapes of the persistent kernel (cells per
+ * wavefront, refill threshold, grid size) on the caller's batch and returns
+ * the fastest in *best (and its time in *best_ms).  C0 is a device copy of
+ * the batch's initial C ([NSPEC][ld]); the batch's C is reset from it before
+ * every trial and once more at the end.  Synchronises the stream.  The
+ * workspace must hold kpp_autotune_workspace_bytes() bytes. */
+size_t kpp_autotune_workspace_bytes(const char* mech, int method, int ncells);
+int kpp_autotune(const char* mech, const kpp_batch* b, const double* C0, const int IPAR[20],
+                 const double RPAR[20], const double* atol, const double* rtol, void* workspace,
+                 size_t workspace_bytes, void* stream, kpp_launch* best, float* best_ms,
+                 int* ntried);
+
+/* ---- graphs: record a sequence of the asynchronous calls above once, replay it ----
+ * No counterpart in KPP.  kpp_capture_begin/kpp_capture_end bracket calls
+ * made on one non-default stream (capture mode thread-local); the recorded
+ * kernels keep the pointers and arguments they were called with, so a replay
+ * integrates whatever those device arrays hold at launch time.
+ * kpp_rosenbrock_dev records only with scalar tolerances (IPAR[1] != 0):
+ * vector tolerances need a host-to-device copy and return KPP_E_ARG under
+ * capture.  Nothing the library records clears memory with a memset node. */
+typedef struct kpp_graph kpp_graph;
+int kpp_capture_begin(void* stream);
+int kpp_capture_end(void* stream, kpp_graph** graph);
+int kpp_graph_launch(kpp_graph* graph, void* stream);
+void kpp_graph_destroy(kpp_graph* graph);
+
 /* ---- learned step scheduler (no counterpart in KPP) ----
  * A 4-layer MLP (in_pad -> 128 -> 128 -> 128 -> 16, GELU) on BF16 matrix
  * cores with FP32 accumulation predicts, from a cell's initial state, the

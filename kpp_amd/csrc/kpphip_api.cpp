@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "mech_ops.h"
@@ -298,12 +299,25 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
   if (!geometry(m, p.coef.S, l, b->ncells, g)) return KPP_E_ARG;
   if (!workspace || workspace_bytes < g.total) return KPP_E_WORKSPACE;
   char* ws = (char*)workspace;
-  const int ntol = (IPAR[1] == 0) ? m->nvar : 1;
   double* dtol = (double*)(ws + kHead);
-  if (hipMemcpyAsync(dtol, atol, sizeof(double) * ntol, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(dtol + m->nvar, rtol, sizeof(double) * ntol, hipMemcpyHostToDevice, st) !=
-          hipSuccess)
-    return KPP_E_HIP;
+  p.vector_tol = (IPAR[1] == 0);
+  if (p.vector_tol) {
+    // vector tolerances go through device memory: a host-to-device copy,
+    // which a stream under graph capture cannot record from pageable memory
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) return KPP_E_HIP;
+    if (cs != hipStreamCaptureStatusNone) return KPP_E_ARG;
+    if (hipMemcpyAsync(dtol, atol, sizeof(double) * m->nvar, hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(dtol + m->nvar, rtol, sizeof(double) * m->nvar, hipMemcpyHostToDevice,
+                       st) != hipSuccess)
+      return KPP_E_HIP;
+    p.atol = dtol;
+    p.rtol = dtol + m->nvar;
+  } else {
+    p.atol0 = atol[0];
+    p.rtol0 = rtol[0];
+  }
   p.ncells = b->ncells;
   p.ldc = b->ld;
   p.C = b->C;
@@ -315,9 +329,6 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
   p.tend = b->tend;
   p.tstart0 = b->tstart0;
   p.tend0 = b->tend0;
-  p.atol = dtol;
-  p.rtol = dtol + m->nvar;
-  p.vector_tol = (IPAR[1] == 0);
   p.autonomous = !(IPAR[0] == 0);
   p.max_steps = (IPAR[2] == 0) ? 100000 : IPAR[2];
   p.Roundoff = unit_roundoff();
@@ -554,3 +565,135 @@ int INTEGRATE_BATCH(kpp_ctx* c, double TIN, double TOUT) {
 }
 
 }  // extern "C"
+
+// ---- graph capture ----
+struct kpp_graph {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" int kpp_capture_begin(void* stream) {
+  if (!stream) return KPP_E_ARG;  // the legacy default stream cannot be captured
+  return hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal) == hipSuccess
+             ? 0
+             : KPP_E_HIP;
+}
+
+extern "C" int kpp_capture_end(void* stream, kpp_graph** out) {
+  if (!stream || !out) return KPP_E_ARG;
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  if (hipStreamEndCapture((hipStream_t)stream, &g) != hipSuccess || !g) return KPP_E_HIP;
+  hipGraphExec_t e = nullptr;
+  if (hipGraphInstantiate(&e, g, nullptr, nullptr, 0) != hipSuccess) {
+    hipGraphDestroy(g);
+    return KPP_E_HIP;
+  }
+  *out = new kpp_graph{g, e};
+  return 0;
+}
+
+extern "C" int kpp_graph_launch(kpp_graph* g, void* stream) {
+  if (!g) return KPP_E_ARG;
+  return hipGraphLaunch(g->exec, (hipStream_t)stream) == hipSuccess ? 0 : KPP_E_HIP;
+}
+
+extern "C" void kpp_graph_destroy(kpp_graph* g) {
+  if (!g) return;
+  hipGraphExecDestroy(g->exec);
+  hipGraphDestroy(g->graph);
+  delete g;
+}
+
+// ---- autotuner ----
+// Candidate launch shapes of the persistent kernel: cells per wavefront
+// (lanes), refill threshold, and grid size relative to the resident capacity.
+namespace {
+struct Cand {
+  int lanes, refill_min, grid_pct;
+};
+const Cand kCands[] = {{64, 1, 100}, {64, 8, 100}, {64, 1, 75},  {64, 1, 125},
+                       {32, 1, 100}, {32, 4, 100}, {48, 1, 100}, {64, 16, 100}};
+constexpr int kNCand = sizeof(kCands) / sizeof(kCands[0]);
+
+bool cand_launch(const MechOps* m, int S, int ncells, const Cand& c, kpp_launch& l) {
+  l.variant = KPP_VARIANT_PERSISTENT;
+  l.lanes = c.lanes;
+  l.refill_min = c.refill_min;
+  l.nblocks = 0;
+  Geometry g;
+  if (!geometry(m, S, &l, ncells, g)) return false;
+  l.nblocks = std::max(1, (int)((long)g.nblocks * c.grid_pct / 100));
+  return true;
+}
+}  // namespace
+
+extern "C" size_t kpp_autotune_workspace_bytes(const char* mech, int method, int ncells) {
+  const MechOps* m = find_mech(mech);
+  RosCoef c;
+  if (!m || !fill_method(method, c) || ncells <= 0) return 0;
+  size_t best = 0;
+  for (int i = 0; i < kNCand; i++) {
+    kpp_launch l;
+    if (!cand_launch(m, c.S, ncells, kCands[i], l)) continue;
+    best = std::max(best, kpp_workspace_bytes(mech, method, &l, ncells));
+  }
+  return best;
+}
+
+extern "C" int kpp_autotune(const char* mech, const kpp_batch* b, const double* C0,
+                            const int IPAR[20], const double RPAR[20], const double* atol,
+                            const double* rtol, void* workspace, size_t workspace_bytes,
+                            void* stream, kpp_launch* best, float* best_ms, int* ntried) {
+  const MechOps* m = find_mech(mech);
+  if (!m) return KPP_E_MECH;
+  if (!b || !C0 || !best || !IPAR || b->ncells <= 0) return KPP_E_ARG;
+  RosCoef coef;
+  if (!fill_method(IPAR[3], coef)) return -2;
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return KPP_E_HIP;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return KPP_E_HIP;
+  }
+  const size_t cbytes = sizeof(double) * (size_t)m->nspec * b->ld;
+  float best_t = 0.f;
+  int tried = 0, rc = 0;
+  for (int i = 0; i < kNCand && rc == 0; i++) {
+    kpp_launch l;
+    if (!cand_launch(m, coef.S, b->ncells, kCands[i], l)) continue;
+    if (kpp_workspace_bytes(mech, IPAR[3], &l, b->ncells) > workspace_bytes) continue;
+    if (hipMemcpyAsync(b->C, C0, cbytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+        hipEventRecord(e0, st) != hipSuccess) {
+      rc = KPP_E_HIP;
+      break;
+    }
+    const int r = kpp_rosenbrock_dev(mech, b, IPAR, RPAR, atol, rtol, &l, workspace,
+                                     workspace_bytes, stream);
+    if (r != 0) {
+      rc = r;
+      break;
+    }
+    float ms = 0.f;
+    if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = KPP_E_HIP;
+      break;
+    }
+    tried++;
+    if (tried == 1 || ms < best_t) {
+      best_t = ms;
+      *best = l;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc == 0 && hipMemcpyAsync(b->C, C0, cbytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    rc = KPP_E_HIP;
+  if (best_ms) *best_ms = best_t;
+  if (ntried) *ntried = tried;
+  if (rc == 0 && tried == 0) rc = KPP_E_WORKSPACE;
+  return rc;
+}
+

@@ -31,8 +31,9 @@ struct RosParams {
   const double* tstart;    // [ncells] or null -> tstart0
   const double* tend;      // [ncells] or null -> tend0
   double tstart0, tend0;
-  const double* atol;      // [NVAR] or [1]
+  const double* atol;      // [NVAR] (vector tolerances) or null
   const double* rtol;
+  double atol0, rtol0;     // scalar tolerances (vector_tol == 0)
   int vector_tol;
   int autonomous;
   int max_steps;

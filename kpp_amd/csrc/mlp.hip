@@ -8,7 +8,6 @@
 // and the order in which cells are launched; the Rosenbrock error control
 // still accepts or rejects every step.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 #include <stdint.h>
 #include <string.h>
 
@@ -234,7 +233,7 @@ __global__ void k_features(int n, int ld, int nspec, const double* __restrict__ 
 // bucket = argmax of the bucket logits; sort key = NBUCKET-1-bucket so the
 // cells predicted to need the most steps are launched first.
 __global__ void k_decode(int n, const float* __restrict__ O, float mu_y, float sd_y, double hmin,
-                         double hmax, double* hstart, int* bucket, uint32_t* key, int* idx) {
+                         double hmax, double* hstart, int* bucket, uint8_t* key) {
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= n) return;
   const float* o = O + (size_t)cell * NOUT;
@@ -247,8 +246,101 @@ __global__ void k_decode(int n, const float* __restrict__ O, float mu_y, float s
   for (int j = 1; j < NBUCKET; j++)
     if (o[1 + j] > o[1 + best]) best = j;
   if (bucket) bucket[cell] = best;
-  if (key) key[cell] = (uint32_t)(NBUCKET - 1 - best);
-  if (idx) idx[cell] = cell;
+  if (key) key[cell] = (uint8_t)(NBUCKET - 1 - best);
+}
+
+// Stable counting sort of cell indices by an 8-valued key, in three kernels
+// with no memory clears (every count is written, none accumulated), so the
+// sequence can be recorded in a graph.  Block b owns cells
+// [b*SORT_CHUNK, (b+1)*SORT_CHUNK).
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 8;
+constexpr int SORT_CHUNK = SORT_THREADS * SORT_ITEMS;
+
+// rank of this thread's key among equal keys of lower threads in the block,
+// and the per-key totals of the block (LDS)
+__device__ __forceinline__ int block_rank(int key, bool valid, int* wave_cnt /*[4][8]*/,
+                                          int* tot /*[8]*/) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int rank = 0;
+#pragma unroll
+  for (int k = 0; k < NBUCKET; k++) {
+    const uint64_t m = __ballot(valid && key == k);
+    if (valid && key == k) rank = __popcll(m & below);
+    if (lane == 0) wave_cnt[wave * NBUCKET + k] = __popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x < NBUCKET) {
+    int t = 0;
+    for (int w = 0; w < SORT_THREADS / 64; w++) t += wave_cnt[w * NBUCKET + threadIdx.x];
+    tot[threadIdx.x] = t;
+  }
+  int before = 0;
+  if (valid)
+    for (int w = 0; w < wave; w++) before += wave_cnt[w * NBUCKET + key];
+  __syncthreads();
+  return rank + before;
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_count(int n, const uint8_t* __restrict__ key,
+                                                             int nblk, int* counts) {
+  __shared__ int wave_cnt[4 * NBUCKET], tot[NBUCKET];
+  int acc = 0;  // thread k < NBUCKET accumulates key k
+  const int base = blockIdx.x * SORT_CHUNK;
+  for (int it = 0; it < SORT_ITEMS; it++) {
+    const int c = base + it * SORT_THREADS + threadIdx.x;
+    const bool valid = c < n;
+    block_rank(valid ? key[c] : 0, valid, wave_cnt, tot);
+    if (threadIdx.x < NBUCKET) acc += tot[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x < NBUCKET) counts[threadIdx.x * nblk + blockIdx.x] = acc;
+}
+
+// exclusive scan of counts[key][block] in key-major order (one block)
+__global__ __launch_bounds__(1024) void k_sort_scan(int total, int* counts) {
+  __shared__ int part[1024];
+  const int per = (total + blockDim.x - 1) / blockDim.x;
+  const int b = threadIdx.x * per;
+  int s = 0;
+  for (int i = b; i < min(total, b + per); i++) s += counts[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int t = 0; t < (int)blockDim.x; t++) {
+      const int v = part[t];
+      part[t] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int run = part[threadIdx.x];
+  for (int i = b; i < min(total, b + per); i++) {
+    const int v = counts[i];
+    counts[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(int n, const uint8_t* __restrict__ key,
+                                                               int nblk, const int* __restrict__ offs,
+                                                               int* order) {
+  __shared__ int wave_cnt[4 * NBUCKET], tot[NBUCKET], run[NBUCKET];
+  if (threadIdx.x < NBUCKET) run[threadIdx.x] = offs[threadIdx.x * nblk + blockIdx.x];
+  __syncthreads();
+  const int base = blockIdx.x * SORT_CHUNK;
+  for (int it = 0; it < SORT_ITEMS; it++) {
+    const int c = base + it * SORT_THREADS + threadIdx.x;
+    const bool valid = c < n;
+    const int k = valid ? key[c] : 0;
+    const int r = block_rank(k, valid, wave_cnt, tot);
+    if (valid) order[run[k] + r] = c;
+    __syncthreads();
+    if (threadIdx.x < NBUCKET) run[threadIdx.x] += tot[threadIdx.x];
+    __syncthreads();
+  }
 }
 
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -453,11 +545,9 @@ int kpp_mlp_train_step(const kpp_mlp* net, const void* X, const float* y, const 
 
 size_t kpp_mlp_schedule_work_bytes(int n) {
   if (n <= 0) return 0;
-  size_t tmp = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                     (const int*)nullptr, (int*)nullptr, n, 0, 3);
-  const size_t a = (size_t)n * 4;
-  return kpp_mlp_work_bytes(n) + 3 * ((a + 255) / 256 * 256) + (tmp + 255) / 256 * 256;
+  const int nblk = (n + SORT_CHUNK - 1) / SORT_CHUNK;
+  return kpp_mlp_work_bytes(n) + ((size_t)n + 255) / 256 * 256 +
+         ((size_t)NBUCKET * nblk * 4 + 255) / 256 * 256;
 }
 
 int kpp_mlp_schedule(const kpp_mlp* net, const void* X, int n, float mu_y, float sd_y,
@@ -469,19 +559,16 @@ int kpp_mlp_schedule(const kpp_mlp* net, const void* X, int n, float mu_y, float
   Work w = carve(work, n);
   forward(st, net, (const __bf16*)X, n, w, false);
   char* p = (char*)work + w.bytes;
-  const size_t a = ((size_t)n * 4 + 255) / 256 * 256;
-  uint32_t* key_in = (uint32_t*)p;
-  uint32_t* key_out = (uint32_t*)(p + a);
-  int* idx_in = (int*)(p + 2 * a);
-  void* tmp = p + 3 * a;
-  size_t tmp_bytes = work_bytes - w.bytes - 3 * a;
+  uint8_t* key = (uint8_t*)p;
+  int* counts = (int*)(p + ((size_t)n + 255) / 256 * 256);
   hipLaunchKernelGGL(k_decode, dim3((n + 255) / 256), dim3(256), 0, st, n, w.O, mu_y, sd_y, hmin,
-                     hmax, hstart, bucket, order ? key_in : nullptr, order ? idx_in : nullptr);
+                     hmax, hstart, bucket, order ? key : nullptr);
   if (order) {
-    // stable: cells of one bucket keep their index order
-    if (hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key_in, key_out, idx_in, order, n, 0,
-                                           3, st) != hipSuccess)
-      return KPP_E_HIP;
+    const int nblk = (n + SORT_CHUNK - 1) / SORT_CHUNK;
+    hipLaunchKernelGGL(k_sort_count, dim3(nblk), dim3(SORT_THREADS), 0, st, n, key, nblk, counts);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, NBUCKET * nblk, counts);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(SORT_THREADS), 0, st, n, key, nblk, counts,
+                       order);
   }
   return hipGetLastError() == hipSuccess ? 0 : KPP_E_HIP;
 }

@@ -366,7 +366,7 @@ struct Cell {
       Ynew[i] = yn;
       const double Ymax = dmax(dabs(y), dabs(yn));
       const double Scale =
-          p.vector_tol ? (p.atol[i] + p.rtol[i] * Ymax) : (p.atol[0] + p.rtol[0] * Ymax);
+          p.vector_tol ? (p.atol[i] + p.rtol[i] * Ymax) : (p.atol0 + p.rtol0 * Ymax);
       Err = Err + (ye * ye) / (Scale * Scale);
     }
     Err = sqrt(Err / (double)NVAR);

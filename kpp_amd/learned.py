@@ -143,11 +143,12 @@ class StepModel:
         return self._work
 
     # ---- inputs ----
-    def features(self, C, temp, sun):
+    def features(self, C, temp, sun, out=None):
         """BF16 inputs [n, in_pad] of the cells of a device batch (C is [NSPEC, ld])."""
         import torch
         n = int(temp.shape[0])
-        X = torch.empty((n, self.in_pad), dtype=torch.bfloat16, device=self.device)
+        X = out if out is not None else torch.empty((n, self.in_pad), dtype=torch.bfloat16,
+                                                    device=self.device)
         _check(_lib().kpp_mlp_features(n, int(C.shape[1]), self.nspec, _p(C), _p(temp), _p(sun),
                                        _p(self.mu), _p(self.sd), self.cfloor, self.in_pad, _p(X),
                                        _stream()), "kpp_mlp_features")
@@ -224,13 +225,18 @@ class StepModel:
                 log(f"[learned] epoch {ep + 1}/{epochs}: last-batch loss {hist[-1]:.4f}")
         return hist
 
-    def schedule(self, X, hmin, hmax, want_order=True):
-        """(hstart float64 [n], bucket int32 [n], order int32 [n] or None) on the device."""
+    def schedule(self, X, hmin, hmax, want_order=True, out=None):
+        """(hstart float64 [n], bucket int32 [n], order int32 [n] or None) on the device.
+
+        ``out``: a previous call's result tuple to write into (no allocation)."""
         import torch
         n = int(X.shape[0])
-        hs = torch.empty(n, dtype=torch.float64, device=self.device)
-        bk = torch.empty(n, dtype=torch.int32, device=self.device)
-        order = torch.empty(n, dtype=torch.int32, device=self.device) if want_order else None
+        if out is not None:
+            hs, bk, order = out
+        else:
+            hs = torch.empty(n, dtype=torch.float64, device=self.device)
+            bk = torch.empty(n, dtype=torch.int32, device=self.device)
+            order = torch.empty(n, dtype=torch.int32, device=self.device) if want_order else None
         ws = self.work(int(_lib().kpp_mlp_schedule_work_bytes(n)))
         net = self.net()
         _check(_lib().kpp_mlp_schedule(ctypes.byref(net), _p(X), n, float(self.mu_y),

@@ -305,3 +305,109 @@ def rates(mech, temp, cfactor, sun, time, out, stream=None):
     n = temp.shape[0]
     _comp(library().kpp_rates_dev, mech.encode(), n, n, _ptr(temp), _ptr(cfactor), _ptr(sun),
           _ptr(time), _ptr(out), _stream_ptr(stream))
+
+
+# ---- autotuner ----
+def autotune(mech: str, C, C0, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, RPAR=None,
+             sun=None, sun_mode=SUN_FIXED, hstart=None, order=None, stream=None):
+    """Fastest persistent-kernel launch shape on this batch: (variant, nblocks,
+    refill_min, lanes), milliseconds, candidates tried.  C is reset from C0."""
+    import torch
+    L = library()
+    _bind_extras(L)
+    n = int(C.shape[1])
+    b = _Batch()
+    b.ncells = n
+    b.ld = n
+    b.C = C.data_ptr()
+    b.temp = temp.data_ptr()
+    b.cfactor = cfactor.data_ptr()
+    b.sun = sun.data_ptr() if sun is not None else None
+    b.sun_mode = sun_mode
+    b.tstart = tstart.data_ptr()
+    b.tend = tend.data_ptr()
+    b.hstart = hstart.data_ptr() if hstart is not None else None
+    b.order = order.data_ptr() if order is not None else None
+    ierr = torch.empty(n, dtype=torch.int32, device=C.device)
+    b.ierr = ierr.data_ptr()
+    IPAR = np.ascontiguousarray(IPAR, np.int32)
+    RPAR = np.zeros(20) if RPAR is None else np.ascontiguousarray(RPAR, np.float64)
+    atol = np.ascontiguousarray(np.atleast_1d(atol), np.float64)
+    rtol = np.ascontiguousarray(np.atleast_1d(rtol), np.float64)
+    nbytes = int(L.kpp_autotune_workspace_bytes(mech.encode(), int(IPAR[3]), n))
+    ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=C.device)
+    best = _Launch()
+    ms = ctypes.c_float()
+    tried = ctypes.c_int()
+    r = L.kpp_autotune(mech.encode(), ctypes.byref(b), ctypes.c_void_p(C0.data_ptr()),
+                       IPAR.ctypes.data, RPAR.ctypes.data, atol.ctypes.data, rtol.ctypes.data,
+                       ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws.numel()),
+                       _stream_ptr(stream), ctypes.byref(best), ctypes.byref(ms),
+                       ctypes.byref(tried))
+    if r != 0:
+        raise KppError(f"kpp_autotune failed with {r}")
+    return ((best.variant, best.nblocks, best.refill_min, best.lanes), float(ms.value),
+            int(tried.value))
+
+
+# ---- graphs ----
+class Graph:
+    """A recorded sequence of library calls (kpp_capture_begin/end); replay with launch()."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def launch(self, stream=None):
+        r = library().kpp_graph_launch(self._h, _stream_ptr(stream))
+        if r != 0:
+            raise KppError(f"kpp_graph_launch failed with {r}")
+
+    def close(self):
+        if self._h:
+            library().kpp_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def capture(fn, stream):
+    """Record the library calls fn() makes on ``stream`` (a non-default torch
+    stream, made current during the call) into a Graph.  Everything fn needs
+    must be allocated beforehand: nothing may allocate while recording."""
+    import torch
+    L = library()
+    _bind_extras(L)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    r = L.kpp_capture_begin(sp)
+    if r != 0:
+        raise KppError(f"kpp_capture_begin failed with {r}")
+    h = ctypes.c_void_p()
+    try:
+        with torch.cuda.stream(stream):
+            fn()
+    finally:
+        r = L.kpp_capture_end(sp, ctypes.byref(h))
+    if r != 0:
+        raise KppError(f"kpp_capture_end failed with {r}")
+    return Graph(h)
+
+
+def _bind_extras(L):
+    if getattr(L, "_extras_bound", False):
+        return
+    vp = ctypes.c_void_p
+    L.kpp_autotune_workspace_bytes.restype = ctypes.c_size_t
+    L.kpp_autotune_workspace_bytes.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+    L.kpp_autotune.argtypes = [ctypes.c_char_p, vp, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp,
+                               vp, vp, vp]
+    L.kpp_capture_begin.argtypes = [vp]
+    L.kpp_capture_end.argtypes = [vp, vp]
+    L.kpp_graph_launch.argtypes = [vp, vp]
+    L.kpp_graph_destroy.argtypes = [vp]
+    L.kpp_graph_destroy.restype = None
+    L._extras_bound = True
+
