@@ -48,10 +48,14 @@ extern "C" {
 /* ---- mechanisms compiled into the library ---- */
 int kpp_mech_count(void);
 const char* kpp_mech_name(int i);
-/* sizes: out[0..7] = NVAR NFIX NSPEC NREACT NONZERO LU_NONZERO
- *        RATES_USE_TIME N_SUN_RATES (ROOT_Parameters.h, src/gen.c:2627-2720) */
-int kpp_mech_info(const char* mech, int out[8]);
+/* sizes: out[0..9] = NVAR NFIX NSPEC NREACT NONZERO LU_NONZERO
+ *        RATES_USE_TIME N_SUN_RATES RATES_USE_STATE N_STATE_RATES
+ * (ROOT_Parameters.h, src/gen.c:2627-2720; the last four describe which rate
+ * constants depend on TIME/SUN and on the concentrations, e.g. MCM's RO2 sum) */
+int kpp_mech_info(const char* mech, int out[10]);
 const char* kpp_mech_species(const char* mech, int k);
+/* Initialize() (src/gen.c:3252-3290): C[NSPEC] = #INITVALUES times CFACTOR */
+int kpp_mech_initial(const char* mech, double* C);
 /* LU_ICOL[LU_NONZERO], LU_CROW[NVAR+1], LU_DIAG[NVAR+1] (ROOT_JacobianSP,
  * src/gen.c:530-588), 0-based */
 int kpp_mech_lu_pattern(const char* mech, int* icol, int* crow, int* diag);
@@ -81,6 +85,8 @@ typedef struct {
   const int* order;      /* device [ncells] processing order (a permutation) or NULL */
   double* hfirst;        /* device [ncells] step log: size of each cell's first accepted step
                             (0 if none was accepted), or NULL.  Not a reference output. */
+  int* fallbacks;        /* device [ncells]: attempts that fell back from the FP32 to the FP64
+                            factorisation (kpp_launch.lu_fp32), or NULL */
 } kpp_batch;
 
 typedef struct {
@@ -89,6 +95,14 @@ typedef struct {
   int refill_min; /* persistent: refill when this many lanes are idle (0 = default 1) */
   int lanes;      /* cells per wavefront, 1..64 (0 = library default: 64, fewer for
                      batches too small to fill the device) */
+  int lu_fp32;    /* 0: FP64 KppDecomp/KppSolve (the reference's arithmetic).
+                     1: mixed precision, no counterpart in KPP: the iteration matrix is
+                     factored in FP32 and every stage solve is an FP32 solve followed by
+                     one FP64 iterative-refinement sweep (FP64 residual against the FP64
+                     matrix, FP32 correction); a cell whose sweep does not converge
+                     (correction above 1e-5 of the solution, or not finite) finishes the
+                     attempt with the FP64 factorisation.  State, stage sums, error norm
+                     and step control stay FP64. */
 } kpp_launch;
 
 /* Device bytes of workspace needed for a launch. */
@@ -120,6 +134,12 @@ int kpp_solve_dev(const char* mech, int n, int ld, const double* JVS, double* X,
  * inside Fun/Jac_SP and are written as 0. */
 int kpp_rates_dev(const char* mech, int n, int ld, const double* temp, const double* cfactor,
                   const double* sun, const double* time, double* RCONST, void* stream);
+/* the same for mechanisms whose rate code reads the concentrations (the F90
+ * rate code inlined into Update_RCONST, e.g. MCM's RO2 sum): C is the device
+ * [NSPEC][ld] state the rates see */
+int kpp_rates_c_dev(const char* mech, int n, int ld, const double* temp, const double* cfactor,
+                    const double* sun, const double* time, const double* C, double* RCONST,
+                    void* stream);
 
 /* ---- host-buffer convenience API (allocates device memory itself) ---- */
 typedef struct kpp_ctx kpp_ctx;

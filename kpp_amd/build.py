@@ -32,10 +32,19 @@ LIB = PKG / "lib"
 ORACLE = REPO / "oracle"
 ORACLE_BUILD = ORACLE / "build"
 
-# mechanism root -> KPP input file, relative to KPP_HOME
+# mechanism root -> KPP input file relative to KPP_HOME, and kppgen options
+# (paths in them are relative to KPP_HOME too)
 MECHANISMS = {
-    "small_strato": "models/small_strato.def",
-    "saprc99": "models/saprc99.def",
+    "small_strato": ("models/small_strato.def", []),
+    "saprc99": ("models/saprc99.def", []),
+    # MCM isoprene export: rate code in Fortran 90 (examples/mcm/constants_mcm.f90,
+    # inlined through #INLINE F90_RCONST), with the air density and its O2, N2,
+    # H2O fractions and the solar zenith angle from examples/mcm/driver_mcm.f90
+    # (cos(zenith) is the cell's SUN)
+    "mcm": ("examples/mcm/mcm.kpp",
+            ["--no-pipelined", "--part-ops", "3000", "--rates-f90", "--f90-module", "examples/mcm/constants_mcm.f90",
+             "--env", "M = CFACTOR*1.0D6; O2 = 0.21*M; N2 = 0.78*M; H2O = 1.0E-2*M; "
+                      "zenith = ACOS(SUN)"]),
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -79,13 +88,24 @@ def generate(mechs=None) -> list[str]:
     exe = build_kppgen()
     GEN.mkdir(parents=True, exist_ok=True)
     done = []
-    for root, rel in (mechs or MECHANISMS).items():
+    for root, (rel, opts) in (mechs or MECHANISMS).items():
         src = home / rel
         if not src.exists():
             raise FileNotFoundError(f"KPP mechanism source {src} not found (set KPP_HOME)")
-        run([str(exe), "--kpp-home", str(home), "--root", root, "-o", str(GEN), str(src)])
+        args = []
+        for k, a in enumerate(opts):
+            args.append(str(home / a) if k > 0 and opts[k - 1] == "--f90-module" else a)
+        run([str(exe), "--kpp-home", str(home), "--root", root, "-o", str(GEN), *args, str(src)])
         (GEN / f"kpphip_{root}.hip").write_text(
             f'#include "{root}_hip.h"\n#include "mech_launch.h"\nKPPHIP_DEFINE_MECH({root})\n')
+        # the mixed-precision integrator and the batched model functions in
+        # translation units of their own (they compile in parallel)
+        (GEN / f"kpphip_{root}_mixed.hip").write_text(
+            f'#include "{root}_hip.h"\n#include "mech_launch.h"\n'
+            f'KPPHIP_DEFINE_MECH_MIXED({root})\n')
+        (GEN / f"kpphip_{root}_comp.hip").write_text(
+            f'#include "{root}_hip.h"\n#include "mech_launch.h"\n'
+            f'KPPHIP_DEFINE_MECH_COMPONENTS({root})\n')
         done.append(root)
     (GEN / "kpphip_registry.inc").write_text("".join(f"KPPHIP_MECH({r})\n" for r in done))
     return done
@@ -100,8 +120,16 @@ def _hipcc_obj(src: Path, out: Path):
 def build_lib(mechs: list[str], jobs: int = 8) -> Path:
     OBJ.mkdir(parents=True, exist_ok=True)
     LIB.mkdir(parents=True, exist_ok=True)
-    srcs = [GEN / f"kpphip_{m}.hip" for m in mechs] + [CSRC / "kpphip_api.cpp",
-                                                        CSRC / "mlp.hip"]
+    srcs = ([GEN / f"kpphip_{m}.hip" for m in mechs] +
+            [GEN / f"kpphip_{m}_mixed.hip" for m in mechs] +
+            [GEN / f"kpphip_{m}_comp.hip" for m in mechs] +
+            [CSRC / "kpphip_api.cpp", CSRC / "mlp.hip"])
+    # largest mechanism first, so the slowest translation units start at once
+    def weight(src):
+        h = GEN / (src.stem.replace("kpphip_", "", 1).replace("_mixed", "").replace("_comp", "")
+                   + "_hip.h")
+        return -(h.stat().st_size if h.exists() else 0)
+    srcs.sort(key=weight)
     objs = [OBJ / (s.stem + ".o") for s in srcs]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_hipcc_obj, srcs, objs))
@@ -110,6 +138,15 @@ def build_lib(mechs: list[str], jobs: int = 8) -> Path:
     run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, out)
     return out
+
+
+def build_driver(lib: Path) -> Path:
+    """The C box-model driver (kpp_amd/drv/box_driver.c) linked to the library."""
+    exe = LIB / "box_driver"
+    run(["gcc", "-O2", "-Wall", "-I", str(REPO / "include"), "-o", str(exe),
+         str(PKG / "drv" / "box_driver.c"), "-L", str(LIB), "-lkpphip",
+         "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-lm"])
+    return exe
 
 
 def build_oracle(mechs: list[str]) -> list[Path]:
@@ -132,7 +169,7 @@ def build_oracle(mechs: list[str]) -> list[Path]:
 def build_all(with_oracle: bool = True) -> dict:
     mechs = generate()
     lib = build_lib(mechs)
-    info = {"lib": str(lib), "mechanisms": mechs}
+    info = {"lib": str(lib), "mechanisms": mechs, "driver": str(build_driver(lib))}
     if with_oracle:
         info["oracle"] = [str(p) for p in build_oracle(mechs)]
     return info

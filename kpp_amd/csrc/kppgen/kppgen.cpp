@@ -32,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <map>
 #include <set>
 #include <sstream>
@@ -700,6 +701,466 @@ struct Printer {
 };
 
 // ---------------------------------------------------------------------------
+// Fortran 90 rate code.  Mechanisms written for KPP's F90 back end (the MCM
+// export, examples/mcm) give their rate expressions, and the code KPP
+// inlines into Update_RCONST, in Fortran.  This translator turns that subset
+// into C with Fortran's typing rules: a real literal without a D exponent or
+// a kind suffix is default REAL (single precision -- KPP's F90 build,
+// util/Makefile_f90:38-39, compiles with gfortran and no -fdefault-real-8),
+// an operation on two single-precision operands is done in single precision,
+// and an operand of kind dp promotes the operation to double.  Variables of
+// the rate code are REAL(dp); INTEGER, PARAMETER constants are integers.
+// ---------------------------------------------------------------------------
+namespace f90 {
+
+enum Ty { TINT = 0, TR4 = 1, TR8 = 2 };
+
+struct Tok {
+  char k;  // 'n' number, 'i' identifier, 'o' operator, 'e' end
+  std::string s;
+};
+
+std::vector<Tok> lex(const std::string& src) {
+  std::vector<Tok> t;
+  size_t i = 0, n = src.size();
+  auto dig = [&](size_t k) { return k < n && isdigit((unsigned char)src[k]); };
+  while (i < n) {
+    char c = src[i];
+    if (isspace((unsigned char)c)) { i++; continue; }
+    if (dig(i) || (c == '.' && dig(i + 1))) {
+      size_t b = i;
+      while (dig(i)) i++;
+      if (i < n && src[i] == '.') {
+        i++;
+        while (dig(i)) i++;
+      }
+      if (i < n && strchr("EeDd", src[i]) &&
+          (dig(i + 1) || ((src[i + 1] == '+' || src[i + 1] == '-') && dig(i + 2)))) {
+        i += 2;
+        while (dig(i)) i++;
+      }
+      if (i < n && src[i] == '_') {
+        i++;
+        while (i < n && (isalnum((unsigned char)src[i]) || src[i] == '_')) i++;
+      }
+      t.push_back({'n', src.substr(b, i - b)});
+      continue;
+    }
+    if (isalpha((unsigned char)c) || c == '_') {
+      size_t b = i;
+      while (i < n && (isalnum((unsigned char)src[i]) || src[i] == '_')) i++;
+      t.push_back({'i', upper(src.substr(b, i - b))});
+      continue;
+    }
+    if (c == '*' && i + 1 < n && src[i + 1] == '*') {
+      t.push_back({'o', "**"});
+      i += 2;
+      continue;
+    }
+    if (strchr("+-*/(),=", c)) {
+      t.push_back({'o', std::string(1, c)});
+      i++;
+      continue;
+    }
+    die("F90 rate code: unexpected character '" + std::string(1, c) + "' in: " + src);
+  }
+  t.push_back({'e', ""});
+  return t;
+}
+
+struct E {
+  std::string c;  // C text
+  Ty t;
+};
+
+// literal -> C text and type
+E number(const std::string& s) {
+  std::string u = upper(s), body = u;
+  bool kind8 = false;
+  size_t us = u.find('_');
+  if (us != std::string::npos) {
+    std::string kind = u.substr(us + 1);
+    body = u.substr(0, us);
+    if (kind == "DP" || kind == "8") kind8 = true;
+    else if (kind != "4" && kind != "SP") die("F90: unknown kind suffix in " + s);
+  }
+  bool dexp = body.find('D') != std::string::npos;
+  bool real = dexp || body.find('.') != std::string::npos || body.find('E') != std::string::npos;
+  if (!real && !kind8) return {body, TINT};
+  for (auto& ch : body)
+    if (ch == 'D') ch = 'E';
+  if (!real) body += ".";
+  if (dexp || kind8) return {body, TR8};
+  return {body + "f", TR4};
+}
+
+struct Ctx {
+  std::map<std::string, long> iparam;            // INTEGER, PARAMETER constants
+  std::map<std::string, int> arrays;             // array name -> declared size
+  std::map<std::string, std::string> inputs;     // TEMP, SUN, ... -> C name
+  std::map<std::string, int> species;            // species name -> index (KPP order)
+  std::function<std::string(int)> conc;          // C text of species k's concentration
+};
+
+struct Parser {
+  std::vector<Tok> t;
+  size_t p = 0;
+  Ctx* cx;
+  std::set<std::string>* uses;
+  std::string src;
+
+  bool op(const char* o) const { return t[p].k == 'o' && t[p].s == o; }
+  void expect(const char* o) {
+    if (!op(o)) die(std::string("F90: expected '") + o + "' in: " + src);
+    p++;
+  }
+  static E bin(const E& a, const std::string& o, const E& b) {
+    return {"(" + a.c + " " + o + " " + b.c + ")", std::max(a.t, b.t)};
+  }
+  static std::string as(const E& a, Ty t) {
+    if (a.t == t) return a.c;
+    return std::string(t == TR8 ? "(double)" : (t == TR4 ? "(float)" : "(int)")) + a.c;
+  }
+  E power(const E& a, const E& b) {
+    if (b.t == TINT) {
+      // integer exponent: repeated multiplication (kpp_powi_*: binary powering)
+      const char* f = a.t == TR8 ? "kpp_powi_d" : (a.t == TR4 ? "kpp_powi_f" : "kpp_powi_i");
+      return {std::string(f) + "(" + a.c + ", " + b.c + ")", a.t};
+    }
+    Ty t = std::max(a.t, b.t);
+    if (t == TR4) return {"powf(" + as(a, TR4) + ", " + b.c + ")", TR4};
+    return {"pow(" + as(a, TR8) + ", " + as(b, TR8) + ")", TR8};
+  }
+  E expr() {
+    E a;
+    if (op("-") || op("+")) {
+      std::string o = t[p++].s;
+      E b = term();
+      a = (o == "-") ? E{"(-" + b.c + ")", b.t} : b;
+    } else {
+      a = term();
+    }
+    while (op("+") || op("-")) {
+      std::string o = t[p++].s;
+      a = bin(a, o, term());
+    }
+    return a;
+  }
+  E term() {
+    E a = factor();
+    while (op("*") || op("/")) {
+      std::string o = t[p++].s;
+      a = bin(a, o, factor());
+    }
+    return a;
+  }
+  E factor() {
+    E a = primary();
+    if (op("**")) {
+      p++;
+      E b = factor();
+      return power(a, b);
+    }
+    return a;
+  }
+  E intrinsic(const std::string& f, std::vector<E>& args) {
+    static const std::set<std::string> one = {"EXP", "LOG",  "LOG10", "SQRT", "COS", "SIN",
+                                              "TAN", "ACOS", "ASIN",  "ATAN", "TANH", "ABS"};
+    if (one.count(f)) {
+      if (args.size() != 1) die("F90: " + f + " takes one argument in: " + src);
+      E a = args[0];
+      if (a.t == TINT && f != "ABS") die("F90: integer argument to " + f + " in: " + src);
+      std::string name = f == "ABS" ? (a.t == TINT ? "abs" : "fabs") : lower(f);
+      if (a.t == TR4) name += "f";
+      return {name + "(" + a.c + ")", a.t};
+    }
+    if (f == "DBLE" && args.size() == 1) return {"((double)" + args[0].c + ")", TR8};
+    if (f == "REAL" && args.size() == 1) return {"((float)" + args[0].c + ")", TR4};
+    if ((f == "MAX" || f == "MIN") && args.size() >= 2) {
+      E a = args[0];
+      for (size_t k = 1; k < args.size(); k++) {
+        Ty ty = std::max(a.t, args[k].t);
+        std::string x = as(a, ty), y = as(args[k], ty);
+        a = {"((" + x + ") " + (f == "MAX" ? ">" : "<") + " (" + y + ") ? (" + x + ") : (" + y +
+                 "))",
+             ty};
+      }
+      return a;
+    }
+    die("F90: unknown function " + f + " in: " + src);
+  }
+  E primary() {
+    const Tok k = t[p];
+    if (k.k == 'n') {
+      p++;
+      return number(k.s);
+    }
+    if (op("(")) {
+      p++;
+      E a = expr();
+      expect(")");
+      return {"(" + a.c + ")", a.t};
+    }
+    if (k.k != 'i') die("F90: unexpected token '" + k.s + "' in: " + src);
+    p++;
+    if (k.s == "C" && op("(")) {  // concentration of a species: C(ind_NAME)
+      p++;
+      if (t[p].k != 'i' || t[p].s.rfind("IND_", 0) != 0) die("F90: C(...) needs ind_NAME: " + src);
+      std::string sp = t[p].s.substr(4);
+      p++;
+      expect(")");
+      auto it = cx->species.find(sp);
+      if (it == cx->species.end()) die("F90: unknown species " + sp + " in: " + src);
+      uses->insert("C(" + sp + ")");
+      return {cx->conc(it->second), TR8};
+    }
+    if (op("(")) {
+      p++;
+      std::vector<E> args;
+      if (!op(")")) {
+        args.push_back(expr());
+        while (op(",")) {
+          p++;
+          args.push_back(expr());
+        }
+      }
+      expect(")");
+      auto ar = cx->arrays.find(k.s);
+      if (ar != cx->arrays.end()) {
+        if (args.size() != 1 || args[0].t != TINT) die("F90: bad index of " + k.s + ": " + src);
+        long ix = strtol(args[0].c.c_str(), nullptr, 10);
+        if (std::to_string(ix) != args[0].c || ix < 1 || ix > ar->second)
+          die("F90: index of " + k.s + " must be a constant in 1.." +
+              std::to_string(ar->second) + ": " + src);
+        uses->insert(k.s + "(" + args[0].c + ")");
+        return {"f_" + k.s + "[" + args[0].c + "]", TR8};
+      }
+      return intrinsic(k.s, args);
+    }
+    auto ip = cx->iparam.find(k.s);
+    if (ip != cx->iparam.end()) return {std::to_string(ip->second), TINT};
+    uses->insert(k.s);
+    auto in = cx->inputs.find(k.s);
+    if (in != cx->inputs.end()) return {in->second, TR8};
+    return {"f_" + k.s, TR8};
+  }
+  static std::string lower(std::string s) {
+    for (auto& ch : s) ch = (char)tolower((unsigned char)ch);
+    return s;
+  }
+};
+
+E translate(const std::string& src, Ctx& cx, std::set<std::string>* uses) {
+  Parser ps{lex(src), 0, &cx, uses, src};
+  E e = ps.expr();
+  if (ps.t[ps.p].k != 'e') die("F90: trailing text in: " + src);
+  return e;
+}
+
+struct Stmt {
+  std::string target;  // "NAME" or "J(4)"
+  std::string ctarget;
+  E rhs;
+  std::set<std::string> uses;
+};
+
+// Statement text -> lines: '!' comments removed, '&' continuations joined.
+std::vector<std::string> logical_lines(const std::string& code) {
+  std::vector<std::string> out;
+  std::string cur;
+  std::istringstream is(code);
+  std::string line;
+  bool cont = false;
+  while (std::getline(is, line)) {
+    size_t bang = line.find('!');
+    if (bang != std::string::npos) line = line.substr(0, bang);
+    size_t b = line.find_first_not_of(" \t\r");
+    if (b == std::string::npos) continue;
+    line = line.substr(b);
+    if (cont && !line.empty() && line[0] == '&') line = line.substr(1);
+    size_t e = line.find_last_not_of(" \t\r");
+    line = line.substr(0, e + 1);
+    bool more = !line.empty() && line.back() == '&';
+    if (more) line.pop_back();
+    cur += line + " ";
+    cont = more;
+    if (!more) {
+      out.push_back(cur);
+      cur.clear();
+    }
+  }
+  if (!cur.empty()) out.push_back(cur);
+  return out;
+}
+
+struct Program {
+  Ctx cx;
+  std::map<std::string, std::vector<std::string>> subs;  // SUBROUTINE -> statement lines
+  std::vector<Stmt> stmts;
+
+  // INTEGER, PARAMETER and array declarations, and subroutine bodies
+  void module(const std::string& text) {
+    std::string in_sub;
+    auto word = [](const std::string& l, size_t from) {
+      size_t b = l.find_first_not_of(" \t", from), e = b;
+      while (e < l.size() && (isalnum((unsigned char)l[e]) || l[e] == '_')) e++;
+      return std::make_pair(upper(b == std::string::npos ? "" : l.substr(b, e - b)), e);
+    };
+    for (auto& l : logical_lines(text)) {
+      std::string u = upper(l);
+      auto [w0, e0] = word(l, 0);
+      if (w0.empty()) continue;
+      if (!in_sub.empty()) {
+        if (w0 == "END") {
+          in_sub.clear();
+          continue;
+        }
+        subs[in_sub].push_back(l);
+        continue;
+      }
+      if (w0 == "SUBROUTINE") {
+        in_sub = word(l, e0).first;
+        subs[in_sub];
+        continue;
+      }
+      size_t dc = u.find("::");
+      if (dc == std::string::npos) continue;
+      std::string decl = u.substr(0, dc), names = l.substr(dc + 2);
+      if (decl.find("INTEGER") != std::string::npos && decl.find("PARAMETER") != std::string::npos) {
+        auto tk = lex(names);
+        for (size_t k = 0; k + 2 < tk.size(); k++)
+          if (tk[k].k == 'i' && tk[k + 1].s == "=") {
+            size_t q = k + 2;
+            bool neg = false;
+            if (tk[q].s == "-") { neg = true; q++; }
+            if (tk[q].k != 'n') die("F90: INTEGER PARAMETER needs a literal: " + l);
+            cx.iparam[tk[k].s] = (neg ? -1 : 1) * strtol(tk[q].s.c_str(), nullptr, 10);
+          }
+      } else if (decl.find("DIMENSION") != std::string::npos) {
+        size_t a = decl.find("DIMENSION");
+        auto tk = lex(decl.substr(a));
+        if (tk.size() < 4 || tk[2].k != 'n') die("F90: DIMENSION needs a literal size: " + l);
+        int size = atoi(tk[2].s.c_str());
+        for (auto& x : lex(names))
+          if (x.k == 'i') cx.arrays[x.s] = size;
+      }
+    }
+  }
+
+  void statement(const std::string& line) {
+    auto tk = lex(line);
+    if (tk[0].k != 'i') die("F90: cannot translate statement: " + line);
+    const std::string& w0 = tk[0].s;
+    if (w0 == "USE" || w0 == "IMPLICIT") return;
+    if (w0 == "CALL") {
+      if (tk[1].k != 'i') die("F90: bad CALL: " + line);
+      auto s = subs.find(tk[1].s);
+      if (s == subs.end()) die("F90: CALL of unknown subroutine " + tk[1].s);
+      for (auto& l : s->second) statement(l);
+      return;
+    }
+    size_t eq = line.find('=');
+    if (eq == std::string::npos) die("F90: cannot translate statement: " + line);
+    std::string lhs = line.substr(0, eq), rhs = line.substr(eq + 1);
+    Stmt st;
+    auto lt = lex(lhs);
+    if (lt.size() == 2 && lt[0].k == 'i') {
+      st.target = lt[0].s;
+      st.ctarget = "f_" + lt[0].s;
+    } else {
+      std::set<std::string> u;
+      E e = translate(lhs, cx, &u);
+      if (u.size() != 1 || e.c.rfind("f_", 0) != 0 || e.c.find('[') == std::string::npos)
+        die("F90: unsupported assignment target: " + line);
+      st.target = *u.begin();
+      st.ctarget = e.c;
+    }
+    st.rhs = translate(rhs, cx, &st.uses);
+    stmts.push_back(st);
+  }
+
+  void code(const std::string& text) {
+    for (auto& l : logical_lines(text)) statement(l);
+  }
+
+  // inputs a name depends on, through the statements before position `upto`
+  std::set<std::string> deps(const std::set<std::string>& uses, size_t upto) const {
+    std::set<std::string> out, todo(uses.begin(), uses.end());
+    std::set<std::pair<std::string, size_t>> seen;
+    std::vector<std::pair<std::string, size_t>> stack;
+    for (auto& u : uses) stack.push_back({u, upto});
+    while (!stack.empty()) {
+      auto [name, pos] = stack.back();
+      stack.pop_back();
+      if (!seen.insert({name, pos}).second) continue;
+      if (name.rfind("C(", 0) == 0) { out.insert("STATE"); continue; }
+      if (cx.inputs.count(name)) { out.insert(name); continue; }
+      size_t k = pos;
+      bool found = false;
+      while (k > 0) {
+        k--;
+        if (stmts[k].target == name) { found = true; break; }
+      }
+      if (!found) die("F90: '" + name + "' is used before it is assigned");
+      for (auto& u : stmts[k].uses) stack.push_back({u, k});
+    }
+    return out;
+  }
+
+  // statements needed to evaluate expressions using `uses` (liveness, in order)
+  std::vector<size_t> needed(std::set<std::string> live) const {
+    std::vector<size_t> keep;
+    for (size_t k = stmts.size(); k-- > 0;) {
+      if (!live.count(stmts[k].target)) continue;
+      keep.push_back(k);
+      live.erase(stmts[k].target);
+      for (auto& u : stmts[k].uses) live.insert(u);
+    }
+    std::reverse(keep.begin(), keep.end());
+    return keep;
+  }
+
+  void emit(std::ostream& o, const std::vector<size_t>& keep, const char* ind = "  ") const {
+    std::set<std::string> declared;
+    for (auto& a : cx.arrays) {
+      bool used = false;
+      for (size_t k : keep)
+        if (stmts[k].target.rfind(a.first + "(", 0) == 0) used = true;
+      if (used) o << ind << "double f_" << a.first << "[" << a.second + 1 << "];\n";
+    }
+    for (size_t k : keep) {
+      const Stmt& s = stmts[k];
+      bool scalar = s.ctarget.find('[') == std::string::npos;
+      if (scalar && declared.insert(s.ctarget).second)
+        o << ind << "double " << s.ctarget << " = " << s.rhs.c << ";\n";
+      else
+        o << ind << s.ctarget << " = " << s.rhs.c << ";\n";
+    }
+  }
+};
+
+const char* kPowiHelpers = R"(
+KPPQ double kpp_powi_d(double x, int n) {
+  double r = 1.0; unsigned m = n < 0 ? -(unsigned)n : (unsigned)n;
+  for (; m; m >>= 1) { if (m & 1u) r *= x; x *= x; }
+  return n < 0 ? 1.0 / r : r;
+}
+KPPQ float kpp_powi_f(float x, int n) {
+  float r = 1.0f; unsigned m = n < 0 ? -(unsigned)n : (unsigned)n;
+  for (; m; m >>= 1) { if (m & 1u) r *= x; x *= x; }
+  return n < 0 ? 1.0f / r : r;
+}
+KPPQ int kpp_powi_i(int x, int n) {
+  int r = 1;
+  if (n < 0) return (x == 1) ? 1 : ((x == -1) ? ((n & 1) ? -1 : 1) : 0);
+  for (; n; n >>= 1) { if (n & 1) r *= x; x *= x; }
+  return r;
+}
+)";
+
+}  // namespace f90
+
+// ---------------------------------------------------------------------------
 // Model code construction shared by both emitters.
 // ---------------------------------------------------------------------------
 struct Model {
@@ -716,6 +1177,26 @@ struct Model {
   std::vector<char> jvs_zero;
   Ops fun_ops, jac_ops, lu_ops, sol_ops;
   std::vector<std::pair<int, std::string>> rconst;  // (reaction, C expression)
+  // rate classes: 0 = fixed per cell, 1 = SUN/TIME, 2 = state (concentrations)
+  std::map<int, int> rate_class;
+  f90::Program* f90 = nullptr;                     // F90 rate code, or null (C syntax)
+  std::map<int, f90::E> f90_rate;                  // translated rate expressions
+  std::map<int, std::set<std::string>> f90_uses;
+
+  // C text of rate j's expression
+  std::string rate_c(int j) const {
+    if (f90) return f90_rate.at(j).c;
+    return m->rates[j].expr;
+  }
+  void classify();
+  // statements of the F90 preamble needed by the given rates
+  std::vector<size_t> preamble(const std::vector<int>& rates) const {
+    std::set<std::string> live;
+    if (!f90) return {};
+    for (int j : rates)
+      for (auto& u : f90_uses.at(j)) live.insert(u);
+    return f90->needed(live);
+  }
 
   Node* rconst_node(int j) {
     auto& r = m->rates[j];
@@ -808,6 +1289,22 @@ bool mentions(const std::string& expr, const std::string& ident) {
     p = e;
   }
   return false;
+}
+
+void Model::classify() {
+  for (auto& pr : rconst) {
+    const int j = pr.first;
+    if (f90) {
+      f90_rate[j] = f90::translate(m->rates[j].expr, f90->cx, &f90_uses[j]);
+      auto d = f90->deps(f90_uses[j], f90->stmts.size());
+      rate_class[j] = d.count("STATE") ? 2 : ((d.count("SUN") || d.count("TIME")) ? 1 : 0);
+    } else {
+      const std::string& e = pr.second;
+      if (e.find("**") != std::string::npos)
+        die("rate of reaction " + std::to_string(j + 1) + " uses '**': pass --rates-f90");
+      rate_class[j] = (mentions(e, "SUN") || mentions(e, "TIME")) ? 1 : 0;
+    }
+  }
 }
 
 // Rate laws of util/UserRateLaws.c:9-78 (parameters are float, arithmetic
@@ -915,13 +1412,21 @@ void emit_c_model(Model& md, const std::string& path) {
   o << "#define KPPQ static\n" << kRateLawsBody << "#undef KPPQ\n";
   o << "/* Update_RCONST (src/gen.c:2343-2473); NUMBER rates are folded into\n"
     << "   Fun/Jac_SP as constants, so their RCONST slot is only informative. */\n";
+  o << "#define KPPQ static\n" << f90::kPowiHelpers << "#undef KPPQ\n";
+  o << "/* V, F: the state the rates see (KPP's global C: the last accepted\n"
+    << "   concentrations); only rate code that reads concentrations uses them. */\n";
   o << "static void kpp_Update_RCONST(double TEMP, double SUN, double CFACTOR, double TIME,\n"
-    << "                              double* RCONST) {\n  (void)SUN; (void)TIME; (void)TEMP; "
-       "(void)CFACTOR;\n";
+    << "                              const double* V, const double* F, double* RCONST) {\n"
+    << "  (void)SUN; (void)TIME; (void)TEMP; (void)CFACTOR; (void)V; (void)F;\n";
+  {
+    std::vector<int> all;
+    for (auto& pr : md.rconst) all.push_back(pr.first);
+    if (md.f90) md.f90->emit(o, md.preamble(all));
+  }
   for (int j = 0; j < S.nreact; j++) {
     auto& r = m.rates[j];
     if (r.type == NUMBER) o << "  RCONST[" << j << "] = " << cfmt((float)r.f) << ";\n";
-    else o << "  RCONST[" << j << "] = (" << r.expr << ");\n";
+    else o << "  RCONST[" << j << "] = (" << md.rate_c(j) << ");\n";
   }
   o << "}\n";
   o << "static void kpp_Fun(const double* V, const double* F, const double* RCT, double* "
@@ -978,6 +1483,8 @@ struct PipeParams {
   int dist = 24, window = 48, chunk = 12;
 };
 PipeParams g_pipe;
+bool g_emit_pipelined = true;
+long g_part_ops = 0;  // split routines into out-of-line parts of this many operations (0: no)
 
 struct PipeOpd {
   int arr, idx, ver;
@@ -1122,24 +1629,41 @@ void emit_hip(Model& md, const std::string& path) {
   o << "static constexpr const char* ROOT = \"" << m.root << "\";\n";
   // rate dependence classes
   bool any_time = false;
-  std::vector<int> rc_const, rc_sun;
+  std::vector<int> rc_const, rc_sun, rc_state;
   for (auto& pr : md.rconst) {
-    bool dep = mentions(pr.second, "SUN") || mentions(pr.second, "TIME");
-    if (mentions(pr.second, "TIME")) any_time = true;
-    (dep ? rc_sun : rc_const).push_back(pr.first);
+    const int j = pr.first, cls = md.rate_class.at(j);
+    if (md.f90) {
+      auto d = md.f90->deps(md.f90_uses.at(j), md.f90->stmts.size());
+      if (d.count("TIME")) any_time = true;
+    } else if (mentions(pr.second, "TIME")) {
+      any_time = true;
+    }
+    (cls == 2 ? rc_state : (cls == 1 ? rc_sun : rc_const)).push_back(j);
   }
   o << "static constexpr bool RATES_USE_TIME = " << (any_time ? "true" : "false") << ";\n";
+  o << "static constexpr bool RATES_USE_STATE = " << (rc_state.empty() ? "false" : "true")
+    << ";\n";
   o << "static constexpr int N_SUN_RATES = " << rc_sun.size() << ";\n";
-  o << "#define KPPQ static __device__ __forceinline__\n" << kRateLawsBody << "#undef KPPQ\n";
-  o << "// rate constants that do not depend on SUN or TIME (once per cell)\n";
+  o << "static constexpr int N_STATE_RATES = " << rc_state.size() << ";\n";
+  o << "#define KPPQ static __device__ __forceinline__\n" << kRateLawsBody << f90::kPowiHelpers
+    << "#undef KPPQ\n";
+  o << "// rate constants that depend on neither SUN, TIME nor the state (once per cell)\n";
   o << "template <class R> static __device__ __forceinline__ void rates_fixed(double TEMP, double "
        "CFACTOR, R RCT) {\n  (void)TEMP; (void)CFACTOR;\n";
-  for (int j : rc_const) o << "  RCT[" << j << "] = (" << m.rates[j].expr << ");\n";
+  if (md.f90) md.f90->emit(o, md.preamble(rc_const));
+  for (int j : rc_const) o << "  RCT[" << j << "] = (" << md.rate_c(j) << ");\n";
   o << "}\n// rate constants that depend on SUN or TIME (every Fun/Jac call)\n";
   o << "template <class R> static __device__ __forceinline__ void rates_sun(double TEMP, double "
        "CFACTOR, double SUN, double TIME, R RCT) {\n  (void)TEMP; (void)CFACTOR; (void)SUN; "
        "(void)TIME;\n";
-  for (int j : rc_sun) o << "  RCT[" << j << "] = (" << m.rates[j].expr << ");\n";
+  if (md.f90) md.f90->emit(o, md.preamble(rc_sun));
+  for (int j : rc_sun) o << "  RCT[" << j << "] = (" << md.rate_c(j) << ");\n";
+  o << "}\n// rate constants that read the concentrations (after every accepted step)\n";
+  o << "template <class TV, class TF, class R> static __device__ __forceinline__ void "
+       "rates_state(double TEMP, double CFACTOR, double SUN, double TIME, TV V, TF F, R RCT) {\n"
+       "  (void)TEMP; (void)CFACTOR; (void)SUN; (void)TIME; (void)V; (void)F;\n";
+  if (md.f90) md.f90->emit(o, md.preamble(rc_state));
+  for (int j : rc_state) o << "  RCT[" << j << "] = (" << md.rate_c(j) << ");\n";
   o << "}\n";
   o << "template <class TV, class TF, class TR, class TO>\nstatic __device__ __forceinline__ void "
        "Fun(TV V, TF F, TR RCT, TO Vdot) {\n";
@@ -1168,10 +1692,6 @@ void emit_hip(Model& md, const std::string& path) {
   // first row (1-based) whose diagonal is exactly zero, the KppDecomp
   // singularity test of util/sutil.c:8, which reads the diagonal before that
   // row is eliminated.
-  auto emit_B = [&](std::ostream& oo) {
-    for (size_t q = 0; q < md.b_expr.size(); q++)
-      oo << "  const double B" << q << " = " << md.b_expr[q] << ";\n";
-  };
   auto bsub = [&](const std::string& e) {
     std::string r;
     for (size_t p = 0; p < e.size(); p++) {
@@ -1186,82 +1706,182 @@ void emit_hip(Model& md, const std::string& path) {
     }
     return r;
   };
-  o << "template <class TV, class TF, class TR, class TJ>\nstatic __device__ __forceinline__ void "
-       "Jac_SP(TV V, TF F, TR RCT, TJ JVS) {\n";
-  emit_B(o);
-  for (int n = 0; n < S.lu_nonzero; n++) o << "  JVS[" << n << "] = " << bsub(md.jvs_expr[n]) << ";\n";
-  o << "}\n";
-  o << "template <class TV, class TF, class TR, class TG>\nstatic __device__ __forceinline__ int "
-       "JacGhimj(TV V, TF F, TR RCT, double ghinv, TG G) {\n";
-  emit_B(o);
-  std::set<int> diag(S.lu_diag.begin(), S.lu_diag.begin() + S.nvar);
-  std::map<int, int> diag_row;
-  for (int i = 0; i < S.nvar; i++) diag_row[S.lu_diag[i]] = i;
-  for (int n = 0; n < S.lu_nonzero; n++) {
-    std::string e = bsub(md.jvs_expr[n]);
-    if (diag.count(n)) {
-      o << "  const double g" << diag_row[n] << " = -(" << e << ")+ghinv;\n  G[" << n << "] = g"
-        << diag_row[n] << ";\n";
-    } else {
-      o << "  G[" << n << "] = -(" << e << ");\n";
+  // Large routines are emitted as a sequence of out-of-line parts of at
+  // most g_part_ops operations (--part-ops): the compiler's time grows
+  // faster than linearly with a function's length, and every value that
+  // crosses a part boundary lives in the accessors' memory anyway.
+  // kind: 0 void, 1 int with early return (first nonzero wins), 2 int whose
+  // parts' results combine as the smallest nonzero value.
+  auto routine = [&](const std::string& tparams, const std::string& name,
+                     const std::string& params, const std::string& args, int kind,
+                     const std::vector<std::pair<std::string, long>>& stmts) {
+    const char* rt = kind ? "int" : "void";
+    std::vector<std::string> parts(1);
+    long ops = 0;
+    for (auto& st : stmts) {
+      if (g_part_ops > 0 && ops >= g_part_ops && !parts.back().empty()) {
+        parts.emplace_back();
+        ops = 0;
+      }
+      parts.back() += st.first;
+      ops += st.second;
     }
+    const char* pre = kind == 2 ? "  int ising = 0;\n" : "";
+    const char* post = kind == 1 ? "  return 0;\n" : (kind == 2 ? "  return ising;\n" : "");
+    if (parts.size() == 1) {
+      o << tparams << "\nstatic __device__ __forceinline__ " << rt << " " << name << "(" << params
+        << ") {\n" << pre << parts[0] << post << "}\n";
+      return;
+    }
+    for (size_t q = 0; q < parts.size(); q++)
+      o << tparams << "\nstatic __device__ __noinline__ " << rt << " " << name << "_p" << q << "("
+        << params << ") {\n" << pre << parts[q] << post << "}\n";
+    o << tparams << "\nstatic __device__ __forceinline__ " << rt << " " << name << "(" << params
+      << ") {\n";
+    if (kind == 1) o << "  int r = 0;\n";
+    if (kind == 2) o << "  int ising = 0, r = 0;\n";
+    for (size_t q = 0; q < parts.size(); q++) {
+      if (kind == 0) o << "  " << name << "_p" << q << "(" << args << ");\n";
+      if (kind == 1) o << "  if ((r = " << name << "_p" << q << "(" << args << "))) return r;\n";
+      if (kind == 2)
+        o << "  r = " << name << "_p" << q << "(" << args << ");\n"
+          << "  if (r && (ising == 0 || r < ising)) ising = r;\n";
+    }
+    o << post << "}\n";
+  };
+  auto nops = [](const std::string& e) {
+    long n = 0;
+    for (char c : e) n += (c == '*' || c == '+' || c == '-' || c == '/');
+    return n + 1;
+  };
+  // Jac_SP and JacGhimj, one Jacobian column at a time: the B terms of
+  // column j (derivatives with respect to variable species j) feed only the
+  // entries of column j, so only one column's B terms are live at once.
+  // Every entry is the same expression, summed in the same order, as KPP's
+  // Jac_SP (src/gen.c:1239-1284).
+  std::vector<std::vector<int>> bcol(S.nvar), ncol(S.nvar);
+  for (auto& kv : md.b_index) bcol[kv.first.second].push_back(kv.second);
+  std::vector<int> lu_row(S.lu_nonzero);
+  for (int i = 0; i < S.nvar; i++)
+    for (int k = S.lu_crow[i]; k < S.lu_crow[i + 1]; k++) lu_row[k] = i;
+  for (int k = 0; k < S.lu_nonzero; k++) ncol[S.lu_icol[k]].push_back(k);
+  for (auto& v : bcol) std::sort(v.begin(), v.end());
+  for (int ghimj = 0; ghimj < 2; ghimj++) {
+    std::vector<std::pair<std::string, long>> st;
+    for (int j = 0; j < S.nvar; j++) {
+      std::string t = "  {\n";
+      long n_ops = 0;
+      for (int q : bcol[j]) {
+        t += "  const double B" + std::to_string(q) + " = " + md.b_expr[q] + ";\n";
+        n_ops += nops(md.b_expr[q]);
+      }
+      for (int n : ncol[j]) {
+        std::string e = bsub(md.jvs_expr[n]);
+        n_ops += nops(e);
+        const int i = lu_row[n];
+        if (!ghimj) {
+          t += "  JVS[" + std::to_string(n) + "] = " + e + ";\n";
+        } else if (n == S.lu_diag[i]) {
+          // G = -Jac, G[diag] += ghinv (int/rosenbrock.c:686-704); the first
+          // (smallest) row whose diagonal is exactly zero is reported, the
+          // KppDecomp singularity test of util/sutil.c:8
+          t += "  { const double g = -(" + e + ")+ghinv; G[" + std::to_string(n) +
+               "] = g; if (g == 0.0 && (ising == 0 || ising > " + std::to_string(i + 1) +
+               ")) ising = " + std::to_string(i + 1) + "; }\n";
+        } else {
+          t += "  G[" + std::to_string(n) + "] = -(" + e + ");\n";
+        }
+      }
+      st.push_back({t + "  }\n", n_ops});
+    }
+    if (ghimj)
+      routine("template <class TV, class TF, class TR, class TG>", "JacGhimj",
+              "TV V, TF F, TR RCT, double ghinv, TG G", "V, F, RCT, ghinv, G", 2, st);
+    else
+      routine("template <class TV, class TF, class TR, class TJ>", "Jac_SP",
+              "TV V, TF F, TR RCT, TJ JVS", "V, F, RCT, JVS", 0, st);
   }
-  o << "  int ising = 0;\n";
-  for (int i = S.nvar - 1; i >= 0; i--) o << "  ising = (g" << i << " == 0.0) ? " << i + 1 << " : ising;\n";
-  o << "  return ising;\n}\n";
-  // Ghimj rebuild from a stored Jacobian (used when Jac is kept)
   // KppDecomp unrolled: row-wise IKJ elimination of util/sutil.c:2-23 with
   // the work row W kept in place (fill closure guarantees every updated
   // column is in the row's pattern).  l = W[j]/U_jj, W[c] -= l*U_jc is
   // bitwise the reference's a=-W[j]/U_jj; W[j]=-a; W[c]+=a*U_jc.
+  // KppDecompChecked adds the singularity test of util/sutil.c:8 at the start
+  // of each row and returns k+1 for the first row whose diagonal is zero.
   for (int checked = 0; checked < 2; checked++) {
-    if (checked)
-      o << "// KppDecomp with the singularity test of util/sutil.c:8 at the start of\n"
-           "// each row; returns k+1 for the first row whose diagonal is exactly zero.\n"
-           "template <class TG>\nstatic __device__ __forceinline__ int KppDecompChecked(TG G) {\n";
-    else
-      o << "template <class TG>\nstatic __device__ __forceinline__ void KppDecomp(TG G) {\n";
+    std::vector<std::pair<std::string, long>> st;
     for (int k = 0; k < S.nvar; k++) {
-      if (checked) o << "  if (G[" << S.lu_diag[k] << "] == 0.0) return " << k + 1 << ";\n";
+      std::string t;
+      long n_ops = 0;
+      if (checked) t += "  if (G[" + std::to_string(S.lu_diag[k]) + "] == 0.0) return " +
+                        std::to_string(k + 1) + ";\n";
       std::map<int, int> pos;  // column -> position in row k
       for (int kk = S.lu_crow[k]; kk < S.lu_crow[k + 1]; kk++) pos[S.lu_icol[kk]] = kk;
       for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
         int j = S.lu_icol[kk];
-        o << "  { const double l = G[" << kk << "]/G[" << S.lu_diag[j] << "]; G[" << kk << "] = l;";
+        t += "  { const auto l = G[" + std::to_string(kk) + "]/G[" + std::to_string(S.lu_diag[j]) +
+             "]; G[" + std::to_string(kk) + "] = l;";
+        n_ops++;
         for (int jj = S.lu_diag[j] + 1; jj < S.lu_crow[j + 1]; jj++) {
           int c = S.lu_icol[jj];
           auto it = pos.find(c);
           if (it == pos.end()) die("LU pattern not closed under elimination");
-          o << " G[" << it->second << "] = G[" << it->second << "] - l*G[" << jj << "];";
+          t += " G[" + std::to_string(it->second) + "] = G[" + std::to_string(it->second) +
+               "] - l*G[" + std::to_string(jj) + "];";
+          n_ops += 2;
         }
-        o << " }\n";
+        t += " }\n";
+      }
+      st.push_back({t, n_ops});
+    }
+    routine("template <class TG>", checked ? "KppDecompChecked" : "KppDecomp", "TG G", "G",
+            checked ? 1 : 0, st);
+  }
+  if (g_emit_pipelined) emit_pipelined(o, S, g_pipe);
+  // Residual of the unfactored matrix: R = B - G*X over the entries of the
+  // pattern that can be nonzero (Jacobian entries and the diagonal; fill-in
+  // entries of Ghimj are exactly zero).  Used by iterative refinement.
+  {
+    std::vector<std::pair<std::string, long>> st;
+    for (int i = 0; i < S.nvar; i++) {
+      std::string t = "  R[" + std::to_string(i) + "] = B[" + std::to_string(i) + "]";
+      long n_ops = 0;
+      for (int k = S.lu_crow[i]; k < S.lu_crow[i + 1]; k++)
+        if (!md.jvs_zero[k] || k == S.lu_diag[i]) {
+          t += " - G[" + std::to_string(k) + "]*X[" + std::to_string(S.lu_icol[k]) + "]";
+          n_ops += 2;
+        }
+      st.push_back({t + ";\n", n_ops});
+    }
+    routine("template <class TG, class TX, class TB, class TR>", "Residual",
+            "TG G, TX X, TB B, TR R", "G, X, B, R", 0, st);
+  }
+  {
+    std::vector<std::pair<std::string, long>> st;
+    for (int i = 0; i < S.nvar; i++) {
+      int b = S.lu_crow[i], e = S.lu_diag[i];
+      if (b < e) {
+        std::string t = "  X[" + std::to_string(i) + "] = X[" + std::to_string(i) + "]";
+        for (int j = b; j < e; j++)
+          t += "-JVS[" + std::to_string(j) + "]*X[" + std::to_string(S.lu_icol[j]) + "]";
+        st.push_back({t + ";\n", 2L * (e - b)});
       }
     }
-    if (checked) o << "  return 0;\n";
-    o << "}\n";
-  }
-  emit_pipelined(o, S, g_pipe);
-  o << "template <class TG, class TX>\nstatic __device__ __forceinline__ void KppSolve(TG JVS, "
-       "TX X) {\n";
-  for (int i = 0; i < S.nvar; i++) {
-    int b = S.lu_crow[i], e = S.lu_diag[i];
-    if (b < e) {
-      o << "  X[" << i << "] = X[" << i << "]";
-      for (int j = b; j < e; j++) o << "-JVS[" << j << "]*X[" << S.lu_icol[j] << "]";
-      o << ";\n";
+    for (int i = S.nvar - 1; i >= 0; i--) {
+      int b = S.lu_diag[i] + 1, e = S.lu_crow[i + 1];
+      std::string t;
+      if (b < e) {
+        t = "  X[" + std::to_string(i) + "] = (X[" + std::to_string(i) + "]";
+        for (int j = b; j < e; j++)
+          t += "-JVS[" + std::to_string(j) + "]*X[" + std::to_string(S.lu_icol[j]) + "]";
+        t += ")/JVS[" + std::to_string(S.lu_diag[i]) + "];\n";
+      } else {
+        t = "  X[" + std::to_string(i) + "] = X[" + std::to_string(i) + "]/JVS[" +
+            std::to_string(S.lu_diag[i]) + "];\n";
+      }
+      st.push_back({t, 2L * (e - b) + 1});
     }
+    routine("template <class TG, class TX>", "KppSolve", "TG JVS, TX X", "JVS, X", 0, st);
   }
-  for (int i = S.nvar - 1; i >= 0; i--) {
-    int b = S.lu_diag[i] + 1, e = S.lu_crow[i + 1];
-    if (b < e) {
-      o << "  X[" << i << "] = (X[" << i << "]";
-      for (int j = b; j < e; j++) o << "-JVS[" << j << "]*X[" << S.lu_icol[j] << "]";
-      o << ")/JVS[" << S.lu_diag[i] << "];\n";
-    } else {
-      o << "  X[" << i << "] = X[" << i << "]/JVS[" << S.lu_diag[i] << "];\n";
-    }
-  }
-  o << "}\n";
   o << "#undef ARR_abc\n#undef ARR_ab\n#undef ARR_ac\n#undef EP2\n#undef EP3\n#undef FALL\n";
   o << "};  // struct " << ns << "\n";
   o << "namespace " << ns << "_data {\n";
@@ -1270,7 +1890,16 @@ void emit_hip(Model& md, const std::string& path) {
   emit_int_array(o, "static const", "LU_DIAG", S.lu_diag);
   o << "static const char* const SPC_NAMES[" << S.nspc << "] = {";
   for (int k = 0; k < S.nspc; k++) o << (k % 8 ? " " : "\n  ") << "\"" << m.spc(m.code[k]).name << "\",";
-  o << "};\n}\n";
+  o << "};\n";
+  // Initialize(): #INITVALUES times CFACTOR (src/gen.c:3252-3290)
+  {
+    auto iv = init_texts(md);
+    o << "static const double CFACTOR = " << efmt(m.cfactor) << ";\n";
+    o << "static const double INIT[" << S.nspc << "] = {";
+    for (int k = 0; k < S.nspc; k++) o << (k % 4 ? " " : "\n  ") << "(" << iv[k] << ")*CFACTOR,";
+    o << "};\n";
+  }
+  o << "}\n";
 }
 
 void emit_stats(Model& md, const std::string& path) {
@@ -1304,8 +1933,9 @@ void emit_stats(Model& md, const std::string& path) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  std::string in, out_dir = ".", root, kpp_home;
-  bool do_reorder = true;
+  std::string in, out_dir = ".", root, kpp_home, env;
+  std::vector<std::string> f90_modules;
+  bool do_reorder = true, rates_f90 = false;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     auto next = [&]() -> std::string {
@@ -1316,11 +1946,18 @@ int main(int argc, char** argv) {
     else if (a == "--root") root = next();
     else if (a == "--kpp-home") kpp_home = next();
     else if (a == "--no-reorder") do_reorder = false;
+    else if (a == "--rates-f90") rates_f90 = true;
+    else if (a == "--no-pipelined") g_emit_pipelined = false;
+    else if (a == "--part-ops") g_part_ops = atol(next().c_str());
+    else if (a == "--f90-module") f90_modules.push_back(next());
+    else if (a == "--env") env = next();
     else if (a == "--pf-dist") g_pipe.dist = atoi(next().c_str());
     else if (a == "--pf-window") g_pipe.window = atoi(next().c_str());
     else if (a == "--pf-chunk") g_pipe.chunk = atoi(next().c_str());
     else if (a == "-h" || a == "--help") {
-      printf("usage: kppgen [--kpp-home DIR] [--root NAME] [-o OUTDIR] mechanism.{def,kpp}\n");
+      printf("usage: kppgen [--kpp-home DIR] [--root NAME] [-o OUTDIR] [--rates-f90\n"
+             "              [--f90-module FILE]... [--env 'F90 statements;...']]\n"
+             "              mechanism.{def,kpp}\n");
       return 0;
     } else in = a;
   }
@@ -1354,6 +1991,25 @@ int main(int argc, char** argv) {
   md.m = &m;
   md.S = reorder(m, do_reorder);
   md.build();
+  f90::Program prog;
+  if (rates_f90) {
+    // inputs of the rate code: the per-cell values of the C ABI
+    prog.cx.inputs = {{"TEMP", "TEMP"}, {"CFACTOR", "CFACTOR"}, {"SUN", "SUN"}, {"TIME", "TIME"}};
+    for (int k = 0; k < md.S.nspc; k++) prog.cx.species[upper(m.spc(m.code[k]).name)] = k;
+    const int nv = md.S.nvar;
+    prog.cx.conc = [nv](int k) {
+      return k < nv ? "V[" + std::to_string(k) + "]" : "F[" + std::to_string(k - nv) + "]";
+    };
+    for (auto& f : f90_modules) prog.module(read_file(f));
+    std::string envc = env;
+    for (auto& ch : envc)
+      if (ch == ';') ch = '\n';
+    prog.code(envc);
+    auto it = sc.inlines.find("F90_RCONST");
+    if (it != sc.inlines.end()) prog.code(it->second);
+    md.f90 = &prog;
+  }
+  md.classify();
   std::string base = out_dir + "/" + root;
   emit_c_model(md, base + "_model.h");
   emit_hip(md, base + "_hip.h");

@@ -12,7 +12,10 @@
 
 #include "mech_ops.h"
 
-#define KPPHIP_MECH(N) extern "C" const kpphip::MechOps* kpphip_mech_##N();
+#define KPPHIP_MECH(N)                                        \
+  extern "C" const kpphip::MechOps* kpphip_mech_##N();        \
+  extern "C" const kpphip::MixedOps* kpphip_mech_mixed_##N(); \
+  extern "C" kpphip::ComponentFn kpphip_mech_comp_##N();
 #include "kpphip_registry.inc"
 #undef KPPHIP_MECH
 
@@ -28,6 +31,29 @@ const MechOps* (*const g_mechs[])() = {
 #undef KPPHIP_MECH
 };
 constexpr int g_nmechs = (int)(sizeof(g_mechs) / sizeof(g_mechs[0]));
+const kpphip::MixedOps* (*const g_mixed[])() = {
+#define KPPHIP_MECH(N) kpphip_mech_mixed_##N,
+#include "kpphip_registry.inc"
+#undef KPPHIP_MECH
+};
+
+kpphip::ComponentFn (*const g_comp[])() = {
+#define KPPHIP_MECH(N) kpphip_mech_comp_##N,
+#include "kpphip_registry.inc"
+#undef KPPHIP_MECH
+};
+
+const kpphip::MixedOps* mixed_of(const MechOps* m) {
+  for (int i = 0; i < g_nmechs; i++)
+    if (g_mechs[i]() == m) return g_mixed[i]();
+  return nullptr;
+}
+
+kpphip::ComponentFn component_of(const MechOps* m) {
+  for (int i = 0; i < g_nmechs; i++)
+    if (g_mechs[i]() == m) return g_comp[i]();
+  return nullptr;
+}
 
 const MechOps* find_mech(const char* name) {
   if (!name) return nullptr;
@@ -164,13 +190,17 @@ struct Geometry {
 };
 
 bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry& g) {
-  const int rows = m->rows(S);
+  const bool mixed = l && l->lu_fp32;
+  const kpphip::MixedOps* mx = mixed ? mixed_of(m) : nullptr;
+  if (mixed && !mx) return false;
+  const int rows = mixed ? mx->rows(S) : m->rows(S);
   if (rows <= 0) return false;
   const int variant = l ? l->variant : KPP_VARIANT_PLAIN;
   if (variant != KPP_VARIANT_PLAIN && variant != KPP_VARIANT_PERSISTENT) return false;
+  if (l && l->lu_fp32 != 0 && l->lu_fp32 != 1) return false;
   int cus = device_cus();
   if (cus <= 0) cus = 256;
-  int occ = m->occupancy(variant, S);
+  int occ = mixed ? mx->occupancy(variant, S) : m->occupancy(variant, S);
   if (occ <= 0) occ = 4;
   const int resident = cus * occ;  // wavefronts the device holds at once
   int lanes = (l && l->lanes > 0) ? l->lanes : 0;
@@ -231,13 +261,23 @@ extern "C" {
 int kpp_mech_count(void) { return g_nmechs; }
 const char* kpp_mech_name(int i) { return (i >= 0 && i < g_nmechs) ? g_mechs[i]()->name : nullptr; }
 
-int kpp_mech_info(const char* mech, int out[8]) {
+int kpp_mech_info(const char* mech, int out[10]) {
   const MechOps* m = find_mech(mech);
   if (!m) return KPP_E_MECH;
   if (!out) return KPP_E_ARG;
   out[0] = m->nvar; out[1] = m->nfix; out[2] = m->nspec; out[3] = m->nreact;
   out[4] = m->nonzero; out[5] = m->lu_nonzero; out[6] = m->rates_use_time;
   out[7] = m->n_sun_rates;
+  out[8] = m->rates_use_state;
+  out[9] = m->n_state_rates;
+  return 0;
+}
+
+int kpp_mech_initial(const char* mech, double* C) {
+  const MechOps* m = find_mech(mech);
+  if (!m) return KPP_E_MECH;
+  if (!C) return KPP_E_ARG;
+  memcpy(C, m->init, sizeof(double) * m->nspec);
   return 0;
 }
 
@@ -349,6 +389,7 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
   p.istat = b->istat;
   p.rstat = b->rstat;
   p.hfirst = b->hfirst;
+  p.fallbacks = b->fallbacks;
   p.order = b->order;
   p.queue = (int*)ws;
   p.rpre = (g.rpre_bytes && !p.rates_given) ? (double*)(ws + kHead + g.tol_bytes) : nullptr;
@@ -356,39 +397,51 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
   p.refill_min = g.refill_min;
   p.lanes = g.lanes;
   const int variant = l ? l->variant : KPP_VARIANT_PLAIN;
-  if (m->launch(p, variant, g.nblocks, st) != hipSuccess) return KPP_E_HIP;
+  const hipError_t le = (l && l->lu_fp32) ? mixed_of(m)->launch(p, variant, g.nblocks, st)
+                                          : m->launch(p, variant, g.nblocks, st);
+  if (le != hipSuccess) return KPP_E_HIP;
   return 0;
 }
 
 static int component(const char* mech, int which, int n, int ld, const double* a, const double* b,
-                     const double* c, const double* d, double* out, int* iout, void* stream) {
+                     const double* c, const double* d, const double* e, double* out, int* iout,
+                     void* stream) {
   const MechOps* m = find_mech(mech);
   if (!m) return KPP_E_MECH;
   if (n < 0 || ld < n) return KPP_E_ARG;
-  if (m->component(which, n, (size_t)ld, a, b, c, d, out, iout, (hipStream_t)stream) != hipSuccess)
+  kpphip::ComponentFn fn = component_of(m);
+  if (!fn || fn(which, n, (size_t)ld, a, b, c, d, e, out, iout, (hipStream_t)stream) !=
+                 hipSuccess)
     return KPP_E_HIP;
   return 0;
 }
 
 int kpp_fun_dev(const char* mech, int n, int ld, const double* V, const double* F,
                 const double* RCT, double* Vdot, void* stream) {
-  return component(mech, KPP_COMP_FUN, n, ld, V, F, RCT, nullptr, Vdot, nullptr, stream);
+  return component(mech, KPP_COMP_FUN, n, ld, V, F, RCT, nullptr, nullptr, Vdot, nullptr, stream);
 }
 int kpp_jac_sp_dev(const char* mech, int n, int ld, const double* V, const double* F,
                    const double* RCT, double* JVS, void* stream) {
-  return component(mech, KPP_COMP_JAC, n, ld, V, F, RCT, nullptr, JVS, nullptr, stream);
+  return component(mech, KPP_COMP_JAC, n, ld, V, F, RCT, nullptr, nullptr, JVS, nullptr, stream);
 }
 int kpp_decomp_dev(const char* mech, int n, int ld, double* JVS, int* ising, void* stream) {
-  return component(mech, KPP_COMP_DECOMP, n, ld, nullptr, nullptr, nullptr, nullptr, JVS, ising,
-                   stream);
+  return component(mech, KPP_COMP_DECOMP, n, ld, nullptr, nullptr, nullptr, nullptr, nullptr, JVS,
+                   ising, stream);
 }
 int kpp_solve_dev(const char* mech, int n, int ld, const double* JVS, double* X, void* stream) {
-  return component(mech, KPP_COMP_SOLVE, n, ld, JVS, nullptr, nullptr, nullptr, X, nullptr,
-                   stream);
+  return component(mech, KPP_COMP_SOLVE, n, ld, JVS, nullptr, nullptr, nullptr, nullptr, X,
+                   nullptr, stream);
 }
 int kpp_rates_dev(const char* mech, int n, int ld, const double* temp, const double* cfactor,
                   const double* sun, const double* time, double* RCONST, void* stream) {
-  return component(mech, KPP_COMP_RATES, n, ld, temp, cfactor, sun, time, RCONST, nullptr,
+  return component(mech, KPP_COMP_RATES, n, ld, temp, cfactor, sun, time, nullptr, RCONST,
+                   nullptr, stream);
+}
+int kpp_rates_c_dev(const char* mech, int n, int ld, const double* temp, const double* cfactor,
+                    const double* sun, const double* time, const double* C, double* RCONST,
+                    void* stream) {
+  if (!C) return KPP_E_ARG;
+  return component(mech, KPP_COMP_RATES, n, ld, temp, cfactor, sun, time, C, RCONST, nullptr,
                    stream);
 }
 
@@ -431,7 +484,7 @@ int kpp_ctx_create(kpp_ctx** out, const char* mech, int ncells, int method, int 
   c->rtol.assign(1, 1.0e-3);
   c->vector_tol = 0;
   const size_t n = (size_t)ncells;
-  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0};
+  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0, 0};
   c->ws_bytes = kpp_workspace_bytes(mech, method, &l, ncells);
   if (c->ws_bytes == 0) { ctx_free(c); return -2; }
   bool ok = hipMalloc(&c->C, sizeof(double) * n * m->nspec) == hipSuccess &&
@@ -488,7 +541,7 @@ static int ctx_run(kpp_ctx* c, const int* IPAR, const double* RPAR, const double
   b.ierr = c->ierr;
   b.istat = c->istat;
   b.rstat = c->rstat;
-  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0};
+  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0, 0};
   int r = kpp_rosenbrock_dev(c->m->name, &b, IPAR, RPAR, atol, rtol, &l, c->ws, c->ws_bytes,
                              nullptr);
   if (hipDeviceSynchronize() != hipSuccess) return KPP_E_HIP;
@@ -618,6 +671,7 @@ constexpr int kNCand = sizeof(kCands) / sizeof(kCands[0]);
 
 bool cand_launch(const MechOps* m, int S, int ncells, const Cand& c, kpp_launch& l) {
   l.variant = KPP_VARIANT_PERSISTENT;
+  l.lu_fp32 = 0;
   l.lanes = c.lanes;
   l.refill_min = c.refill_min;
   l.nblocks = 0;

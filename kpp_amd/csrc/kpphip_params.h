@@ -48,6 +48,7 @@ struct RosParams {
   int* istat;              // [8][ldc] or null
   double* rstat;           // [2][ldc] (Texit, Hexit) or null
   double* hfirst;          // [ncells] size of each cell's first accepted step, or null
+  int* fallbacks;          // [ncells] attempts that fell back to the FP64 LU (mixed), or null
   const int* order;        // [ncells] launch order (cell permutation) or null
   double* scratch;         // tiles
   int* queue;              // persistent kernel: next-cell counter (starts at 0)

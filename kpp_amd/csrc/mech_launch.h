@@ -53,76 +53,82 @@ __global__ __launch_bounds__(256) void k_solve(int n, size_t ld, const double* J
 template <class M>
 __global__ __launch_bounds__(256) void k_rates(int n, size_t ld, const double* temp,
                                                const double* cfactor, const double* sun,
-                                               const double* time, double* R) {
+                                               const double* time, const double* C, double* R) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   Strided r{R + c, ld};
   for (int i = 0; i < M::NREACT; i++) r[i] = 0.0;
+  const double tm = time ? time[c] : 0.0;
   M::rates_fixed(temp[c], cfactor[c], r);
-  M::rates_sun(temp[c], cfactor[c], sun[c], time ? time[c] : 0.0, r);
+  M::rates_sun(temp[c], cfactor[c], sun[c], tm, r);
+  if (M::RATES_USE_STATE && C)
+    M::rates_state(temp[c], cfactor[c], sun[c], tm, Strided{const_cast<double*>(C) + c, ld},
+                   Strided{const_cast<double*>(C) + (size_t)M::NVAR * ld + c, ld}, r);
 }
 
 static __global__ void k_zero_int(int* q) { *q = 0; }
 
-template <class M, int S>
+template <class M, int S, bool MIX>
 hipError_t launch_s(const RosParams& p, int variant, int nblocks, hipStream_t st) {
   if (variant == KPP_VARIANT_PERSISTENT) {
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, p.queue);
     if (p.rpre)
       hipLaunchKernelGGL(ros_rates_prepass<M>, dim3((p.ncells + 255) / 256), dim3(256), 0, st, p);
-    hipLaunchKernelGGL((ros_persistent_kernel<M, S>), dim3(nblocks), dim3(TILE), 0, st, p);
+    hipLaunchKernelGGL((ros_persistent_kernel<M, S, MIX>), dim3(nblocks), dim3(TILE), 0, st, p);
   } else {
-    hipLaunchKernelGGL((ros_plain_kernel<M, S>), dim3(nblocks), dim3(TILE), 0, st, p);
+    hipLaunchKernelGGL((ros_plain_kernel<M, S, MIX>), dim3(nblocks), dim3(TILE), 0, st, p);
   }
   return hipGetLastError();
 }
 
-template <class M>
+template <class M, bool MIX>
 hipError_t launch_mech(const RosParams& p, int variant, int nblocks, hipStream_t st) {
   switch (p.coef.S) {
-    case 2: return launch_s<M, 2>(p, variant, nblocks, st);
-    case 3: return launch_s<M, 3>(p, variant, nblocks, st);
-    case 4: return launch_s<M, 4>(p, variant, nblocks, st);
-    case 6: return launch_s<M, 6>(p, variant, nblocks, st);
+    case 2: return launch_s<M, 2, MIX>(p, variant, nblocks, st);
+    case 3: return launch_s<M, 3, MIX>(p, variant, nblocks, st);
+    case 4: return launch_s<M, 4, MIX>(p, variant, nblocks, st);
+    case 6: return launch_s<M, 6, MIX>(p, variant, nblocks, st);
   }
   return hipErrorInvalidValue;
 }
 
 // Resident wavefronts per CU of a kernel (its register/LDS limits).
-template <class M, int S>
+template <class M, int S, bool MIX>
 int occupancy_s(int variant) {
   int n = 0;
-  hipError_t e = (variant == KPP_VARIANT_PERSISTENT)
-      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_persistent_kernel<M, S>, TILE, 0)
-      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_plain_kernel<M, S>, TILE, 0);
+  hipError_t e =
+      (variant == KPP_VARIANT_PERSISTENT)
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_persistent_kernel<M, S, MIX>,
+                                                         TILE, 0)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_plain_kernel<M, S, MIX>, TILE, 0);
   return e == hipSuccess ? n : 0;
 }
-template <class M>
+template <class M, bool MIX>
 int occupancy_mech(int variant, int S) {
   switch (S) {
-    case 2: return occupancy_s<M, 2>(variant);
-    case 3: return occupancy_s<M, 3>(variant);
-    case 4: return occupancy_s<M, 4>(variant);
-    case 6: return occupancy_s<M, 6>(variant);
+    case 2: return occupancy_s<M, 2, MIX>(variant);
+    case 3: return occupancy_s<M, 3, MIX>(variant);
+    case 4: return occupancy_s<M, 4, MIX>(variant);
+    case 6: return occupancy_s<M, 6, MIX>(variant);
+  }
+  return 0;
+}
+
+template <class M, bool MIX>
+int rows_mech(int S) {
+  switch (S) {
+    case 2: return Layout<M, 2, MIX>::ROWS;
+    case 3: return Layout<M, 3, MIX>::ROWS;
+    case 4: return Layout<M, 4, MIX>::ROWS;
+    case 6: return Layout<M, 6, MIX>::ROWS;
   }
   return 0;
 }
 
 template <class M>
-int rows_mech(int S) {
-  switch (S) {
-    case 2: return Layout<M, 2>::ROWS;
-    case 3: return Layout<M, 3>::ROWS;
-    case 4: return Layout<M, 4>::ROWS;
-    case 6: return Layout<M, 6>::ROWS;
-  }
-  return -1;
-}
-
-template <class M>
 hipError_t component_mech(int which, int n, size_t ld, const double* a, const double* b,
-                          const double* c, const double* d, double* out, int* iout,
-                          hipStream_t st) {
+                          const double* c, const double* d, const double* e, double* out,
+                          int* iout, hipStream_t st) {
   const int bs = 256, nb = (n + bs - 1) / bs;
   if (n <= 0) return hipSuccess;
   switch (which) {
@@ -131,7 +137,7 @@ hipError_t component_mech(int which, int n, size_t ld, const double* a, const do
     case KPP_COMP_DECOMP: hipLaunchKernelGGL(k_decomp<M>, dim3(nb), dim3(bs), 0, st, n, ld, out, iout); break;
     case KPP_COMP_SOLVE: hipLaunchKernelGGL(k_solve<M>, dim3(nb), dim3(bs), 0, st, n, ld, a, out); break;
     case KPP_COMP_RATES:
-      hipLaunchKernelGGL(k_rates<M>, dim3(nb), dim3(bs), 0, st, n, ld, a, b, c, d, out);
+      hipLaunchKernelGGL(k_rates<M>, dim3(nb), dim3(bs), 0, st, n, ld, a, b, c, d, e, out);
       break;
     default: return hipErrorInvalidValue;
   }
@@ -150,14 +156,16 @@ MechOps make_ops(const char* name) {
   o.lu_nonzero = M::LU_NONZERO;
   o.rates_use_time = M::RATES_USE_TIME ? 1 : 0;
   o.n_sun_rates = M::N_SUN_RATES;
+  o.rates_use_state = M::RATES_USE_STATE ? 1 : 0;
+  o.n_state_rates = M::N_STATE_RATES;
   o.species = D::names();
   o.lu_icol = D::icol();
   o.lu_crow = D::crow();
   o.lu_diag = D::diag();
-  o.rows = rows_mech<M>;
-  o.occupancy = occupancy_mech<M>;
-  o.launch = launch_mech<M>;
-  o.component = component_mech<M>;
+  o.init = D::init();
+  o.rows = rows_mech<M, false>;
+  o.occupancy = occupancy_mech<M, false>;
+  o.launch = launch_mech<M, false>;
   return o;
 }
 
@@ -171,9 +179,25 @@ MechOps make_ops(const char* name) {
     static const int* icol() { return kppm_##NAME##_data::LU_ICOL; }                    \
     static const int* crow() { return kppm_##NAME##_data::LU_CROW; }                    \
     static const int* diag() { return kppm_##NAME##_data::LU_DIAG; }                    \
+    static const double* init() { return kppm_##NAME##_data::INIT; }                    \
   };                                                                                    \
   }                                                                                     \
   extern "C" const kpphip::MechOps* kpphip_mech_##NAME() {                              \
     static const kpphip::MechOps ops = kpphip::make_ops<kppm_##NAME, Data_##NAME>(#NAME); \
     return &ops;                                                                        \
+  }
+
+// Defines kpphip_mech_mixed_<NAME>(): the mixed-precision integrator of NAME.
+#define KPPHIP_DEFINE_MECH_MIXED(NAME)                                                  \
+  extern "C" const kpphip::MixedOps* kpphip_mech_mixed_##NAME() {                       \
+    static const kpphip::MixedOps ops = {kpphip::rows_mech<kppm_##NAME, true>,          \
+                                         kpphip::occupancy_mech<kppm_##NAME, true>,     \
+                                         kpphip::launch_mech<kppm_##NAME, true>};       \
+    return &ops;                                                                        \
+  }
+
+// Defines kpphip_mech_comp_<NAME>(): the batched model functions of NAME.
+#define KPPHIP_DEFINE_MECH_COMPONENTS(NAME)                                              \
+  extern "C" kpphip::ComponentFn kpphip_mech_comp_##NAME() {                             \
+    return kpphip::component_mech<kppm_##NAME>;                                          \
   }

@@ -19,15 +19,26 @@ namespace kpphip {
 struct MechOps {
   const char* name;
   int nvar, nfix, nspec, nreact, nonzero, lu_nonzero;
-  int rates_use_time, n_sun_rates;
+  int rates_use_time, n_sun_rates, rates_use_state, n_state_rates;
   const char* const* species;
   const int *lu_icol, *lu_crow, *lu_diag;
+  const double* init;  // Initialize(): #INITVALUES times CFACTOR, [NSPEC]
   int (*rows)(int S);  // tile rows per lane for S stages
   int (*occupancy)(int variant, int S);  // resident wavefronts per CU (0 if unknown)
   hipError_t (*launch)(const RosParams& p, int variant, int nblocks, hipStream_t st);
-  hipError_t (*component)(int which, int n, size_t ld, const double* a, const double* b,
-                          const double* c, const double* d, double* out, int* iout,
-                          hipStream_t st);
+};
+
+// Batched model functions of a mechanism (their own translation unit).
+typedef hipError_t (*ComponentFn)(int which, int n, size_t ld, const double* a, const double* b,
+                                  const double* c, const double* d, const double* e, double* out,
+                                  int* iout, hipStream_t st);
+
+// The mixed-precision integrator of a mechanism (its own translation unit):
+// FP32 LU and triangular solves with one FP64 refinement sweep per solve.
+struct MixedOps {
+  int (*rows)(int S);
+  int (*occupancy)(int variant, int S);
+  hipError_t (*launch)(const RosParams& p, int variant, int nblocks, hipStream_t st);
 };
 
 }  // namespace kpphip

@@ -73,9 +73,17 @@ struct Acc {
   gdouble* __restrict__ b;
   __device__ __forceinline__ gdouble& operator[](int i) const { return b[i * TILE]; }
 };
+using gfloat = __attribute__((address_space(1))) float;
+struct AccF {
+  gfloat* __restrict__ b;
+  __device__ __forceinline__ gfloat& operator[](int i) const { return b[i * TILE]; }
+};
 
-// Row layout of a tile for mechanism M and S stages.
-template <class M, int S>
+// Row layout of a tile for mechanism M and S stages (a row holds one double
+// per lane).  The mixed-precision integrator (MIX) adds an FP32 copy of the
+// iteration matrix (two FP32 elements per row), an FP32 solve vector, and the
+// FP64 first solution and residual of the refinement sweep.
+template <class M, int S, bool MIX = false>
 struct Layout {
   static constexpr int Y = 0;
   static constexpr int FIX = Y + M::NVAR;
@@ -86,8 +94,18 @@ struct Layout {
   static constexpr int YNEW = DFDT + M::NVAR;
   static constexpr int K = YNEW + M::NVAR;
   static constexpr int G = K + S * M::NVAR;
-  static constexpr int ROWS = G + M::LU_NONZERO;
+  static constexpr int G32 = G + M::LU_NONZERO;                 // FP32 rows
+  static constexpr int X32 = G32 + (M::LU_NONZERO + 1) / 2;     // FP32 rows
+  static constexpr int XR = X32 + (M::NVAR + 1) / 2;
+  static constexpr int RR = XR + M::NVAR;
+  static constexpr int ROWS = MIX ? RR + M::NVAR : G + M::LU_NONZERO;
 };
+
+// Refinement of the FP32 solves: the correction d of the one FP64 sweep must
+// satisfy ||d||_inf <= kRefineTol * ||x||_inf, else the cell's attempt goes
+// on with the FP64 factorisation (the sweep's contraction factor is about
+// ||d||/||x||, so the refined solution is then accurate to ~1e-10).
+constexpr double kRefineTol = 1e-5;
 
 // util/UpdateSun.c:1-20
 __device__ __forceinline__ double update_sun(double TIME) {
@@ -146,10 +164,28 @@ KPPHIP_MODEL_CALL void call_solve(dptr G, dptr X) {
   M::KppSolve(Acc{G}, Acc{X});
 #endif
 }
+using fptr = gfloat* __restrict__;
+template <class M>
+KPPHIP_MODEL_CALL void call_decomp32(fptr G) {
+  M::KppDecomp(AccF{G});
+}
+template <class M>
+KPPHIP_MODEL_CALL void call_solve32(fptr G, fptr X) {
+  M::KppSolve(AccF{G}, AccF{X});
+}
+template <class M>
+KPPHIP_MODEL_CALL void call_residual(dptr G, dptr X, dptr B, dptr R) {
+  M::Residual(Acc{G}, Acc{X}, Acc{B}, Acc{R});
+}
 template <class M>
 KPPHIP_MODEL_CALL void call_rates_sun(double TEMP, double CFACTOR, double SUN, double TIME,
                                       dptr R) {
   M::rates_sun(TEMP, CFACTOR, SUN, TIME, Acc{R});
+}
+template <class M>
+KPPHIP_MODEL_CALL void call_rates_state(double TEMP, double CFACTOR, double SUN, double TIME,
+                                        dptr V, dptr F, dptr R) {
+  M::rates_state(TEMP, CFACTOR, SUN, TIME, Acc{V}, Acc{F}, Acc{R});
 }
 template <class M>
 KPPHIP_MODEL_CALL void call_rates_fixed(double TEMP, double CFACTOR, dptr R) {
@@ -164,26 +200,35 @@ struct CellState {
   int Direction;
   bool RejectLastH, RejectMoreH, newstep;
   int Nfun, Njac, Nstp, Nacc, Nrej, Ndec, Nsol, Nsng;
+  int Nfb;  // attempts that fell back from the FP32 to the FP64 factorisation
 };
 
-template <class M, int S>
+template <class M, int S, bool MIX = false>
 struct Cell {
-  using L = Layout<M, S>;
+  using L = Layout<M, S, MIX>;
   double* base;  // lane base pointer inside its tile
   __device__ __forceinline__ Acc at(int row) const { return Acc{(gdouble*)(base + row * TILE)}; }
+  // FP32 rows: element i of this lane at float index 2*row*TILE + i*TILE
+  __device__ __forceinline__ AccF at32(int row) const {
+    return AccF{(gfloat*)((float*)(base - lane_of()) + 2 * row * TILE + lane_of())};
+  }
+  __device__ __forceinline__ int lane_of() const { return (int)(threadIdx.x & (TILE - 1)); }
 
   // Rate constants at time T (int/rosenbrock.c:1257-1296 call Update_SUN and
   // Update_RCONST before every Fun/Jac).  Rates free of SUN and TIME were
   // computed once per cell with the same inputs, so not recomputing them
-  // changes no value.
-  __device__ __forceinline__ void rates_at(const RosParams& p, const CellState& s,
-                                           double T) const {
+  // changes no value.  Rates that read the concentrations see the last
+  // accepted state (KPP's global C, which Rosenbrock updates in place on
+  // acceptance), so between acceptances they are recomputed only when SUN or
+  // TIME can have changed; `accepted` marks the first call after a new state.
+  __device__ __forceinline__ void rates_at(const RosParams& p, const CellState& s, double T,
+                                           bool accepted) const {
     if (p.rates_given) return;
-    if (p.sun_mode == KPP_SUN_TOD) {
-      call_rates_sun<M>(s.TEMP, s.CFACTOR, update_sun(T), T, at(L::RCT).b);
-    } else if (M::RATES_USE_TIME) {
-      call_rates_sun<M>(s.TEMP, s.CFACTOR, s.SUN, T, at(L::RCT).b);
-    }
+    const bool tod = p.sun_mode == KPP_SUN_TOD;
+    const double sun = tod ? update_sun(T) : s.SUN;
+    if (tod || M::RATES_USE_TIME) call_rates_sun<M>(s.TEMP, s.CFACTOR, sun, T, at(L::RCT).b);
+    if (M::RATES_USE_STATE && (accepted || tod || M::RATES_USE_TIME))
+      call_rates_state<M>(s.TEMP, s.CFACTOR, sun, T, at(L::Y).b, at(L::FIX).b, at(L::RCT).b);
   }
 
   // Load a cell into this lane's slot; the set-up part of Rosenbrock()
@@ -210,6 +255,7 @@ struct Cell {
     }
     s.Nfun = p.stat0[0]; s.Njac = p.stat0[1]; s.Nstp = p.stat0[2]; s.Nacc = p.stat0[3];
     s.Nrej = p.stat0[4]; s.Ndec = p.stat0[5]; s.Nsol = p.stat0[6]; s.Nsng = p.stat0[7];
+    s.Nfb = 0;
     s.Tstart = p.tstart ? p.tstart[cell] : p.tstart0;
     s.Tend = p.tend ? p.tend[cell] : p.tend0;
     const double span = abs_diff_macro(s.Tend, s.Tstart);
@@ -242,6 +288,7 @@ struct Cell {
       p.rstat[cell] = Texit;
       p.rstat[ld + cell] = s.Hexit;
     }
+    if (p.fallbacks) p.fallbacks[cell] = s.Nfb;
   }
 
   // One step attempt.  Returns 0 while the cell continues, else its IERR
@@ -265,7 +312,7 @@ struct Cell {
       s.Hexit = s.H;
       s.H = dmin(s.H, abs_diff_macro(s.Tend, s.T));
       // Fcn0 = F(T, Y)
-      rates_at(p, s, s.T);
+      rates_at(p, s, s.T, true);
       call_fun<M>(Y.b, F.b, R.b, Fcn0.b);
       s.Nfun++;
       // dF/dT by forward difference (int/rosenbrock.c:663-682)
@@ -273,7 +320,7 @@ struct Cell {
         s.Nfun++;
         if (!skip_dfdt) {
           const double Delta = sqrt(Roundoff) * dmax(KPP_DELTAMIN, dabs(s.T));
-          rates_at(p, s, s.T + Delta);
+          rates_at(p, s, s.T + Delta, false);
           call_fun<M>(Y.b, F.b, R.b, dFdT.b);
           const double inv = 1.0 / Delta;
           for (int i = 0; i < NVAR; i++) {
@@ -290,7 +337,7 @@ struct Cell {
 
     // ros_PrepareMatrix: Ghimj = -Jac0 + I/(H*gamma1), KppDecomp, halve H on
     // an exactly zero diagonal, fail after 5 consecutive singular matrices.
-    rates_at(p, s, s.T);
+    rates_at(p, s, s.T, false);
     int Nconsecutive = 0;
     for (;;) {
       const double ghinv = 1.0 / (Dir * s.H * m.Gamma[0]);
@@ -306,7 +353,16 @@ struct Cell {
         return -8;
       }
     }
-    call_decomp<M>(G.b);
+    // FP64: KppDecomp in place.  MIX: factor an FP32 copy and keep the FP64
+    // matrix for the residuals of the refinement sweeps.
+    bool lu64 = !MIX;
+    if (MIX) {
+      AccF G32 = at32(L::G32);
+      for (int i = 0; i < M::LU_NONZERO; i++) G32[i] = (float)G[i];
+      call_decomp32<M>(G32.b);
+    } else {
+      call_decomp<M>(G.b);
+    }
 
     const double H = s.H;
 #pragma unroll
@@ -329,7 +385,7 @@ struct Cell {
           Ynew[i] = y;
         }
         const double Tau = s.T + m.Alpha[istage - 1] * Dir * H;
-        rates_at(p, s, Tau);
+        rates_at(p, s, Tau, false);
         call_fun<M>(Ynew.b, F.b, R.b, Fcn.b);
         s.Nfun++;
       }
@@ -349,7 +405,36 @@ struct Cell {
         if (use_g) k = k + HG * dFdT[i];
         Ki[i] = k;
       }
-      call_solve<M>(G.b, Ki.b);
+      if (MIX && !lu64) {
+        // x1 = FP32 solve of b; r = b - A x1 in FP64; x = x1 + FP32 solve of r
+        AccF G32 = at32(L::G32), X32 = at32(L::X32);
+        Acc XR = at(L::XR), RR = at(L::RR);
+        for (int i = 0; i < NVAR; i++) X32[i] = (float)Ki[i];
+        call_solve32<M>(G32.b, X32.b);
+        for (int i = 0; i < NVAR; i++) XR[i] = (double)X32[i];
+        call_residual<M>(G.b, XR.b, Ki.b, RR.b);
+        for (int i = 0; i < NVAR; i++) X32[i] = (float)RR[i];
+        call_solve32<M>(G32.b, X32.b);
+        double dmx = 0.0, xmx = 0.0;
+        for (int i = 0; i < NVAR; i++) {
+          const double d = (double)X32[i], x = XR[i] + d;
+          XR[i] = x;
+          dmx = dmax(dmx, dabs(d));
+          xmx = dmax(xmx, dabs(x));
+        }
+        if (dmx <= kRefineTol * xmx) {
+          for (int i = 0; i < NVAR; i++) Ki[i] = XR[i];
+        } else {
+          // the sweep did not converge (or produced a non-finite value):
+          // this and the remaining stages use the FP64 factorisation
+          lu64 = true;
+          s.Nfb++;
+          call_decomp<M>(G.b);
+          call_solve<M>(G.b, Ki.b);
+        }
+      } else {
+        call_solve<M>(G.b, Ki.b);
+      }
       s.Nsol++;
     }
     // new solution, error estimate and ros_ErrorNorm (:582-591, :632-659)
@@ -420,13 +505,13 @@ __global__ __launch_bounds__(256) void ros_rates_prepass(RosParams p) {
 // Plain kernel: one lane per cell for the whole integration.  The first
 // p.lanes lanes of each wavefront carry cells (p.lanes < 64 puts more
 // wavefronts in flight for small batches).
-template <class M, int S>
+template <class M, int S, bool MIX = false>
 __global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void ros_plain_kernel(RosParams p) {
-  using L = Layout<M, S>;
+  using L = Layout<M, S, MIX>;
   if ((int)threadIdx.x >= p.lanes) return;
   const int k = blockIdx.x * p.lanes + threadIdx.x;
   if (k >= p.ncells) return;
-  Cell<M, S> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + threadIdx.x};
+  Cell<M, S, MIX> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + threadIdx.x};
   CellState s;
   c.start(p, p.order ? p.order[k] : k, s);
   while (c.attempt(p, s) == 0) {
@@ -440,11 +525,11 @@ __global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void ros_plain_kernel(RosPar
 // single atomic and the idle lanes take consecutive entries.  A wavefront
 // leaves once the queue is exhausted and none of its lanes is active, so
 // every wavefront reaches the exit.  Lanes at or above p.lanes never work.
-template <class M, int S>
+template <class M, int S, bool MIX = false>
 __global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void ros_persistent_kernel(RosParams p) {
-  using L = Layout<M, S>;
+  using L = Layout<M, S, MIX>;
   const int lane = threadIdx.x;
-  Cell<M, S> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + lane};
+  Cell<M, S, MIX> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + lane};
   CellState s;
   const bool worker = lane < p.lanes;
   bool active = false;

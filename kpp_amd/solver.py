@@ -59,12 +59,13 @@ class _Batch(ctypes.Structure):
         ("hstart", ctypes.c_void_p), ("rconst", ctypes.c_void_p),
         ("ierr", ctypes.c_void_p), ("istat", ctypes.c_void_p), ("rstat", ctypes.c_void_p),
         ("order", ctypes.c_void_p), ("hfirst", ctypes.c_void_p),
+        ("fallbacks", ctypes.c_void_p),
     ]
 
 
 class _Launch(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("nblocks", ctypes.c_int), ("refill_min", ctypes.c_int),
-                ("lanes", ctypes.c_int)]
+                ("lanes", ctypes.c_int), ("lu_fp32", ctypes.c_int)]
 
 
 _lib = None
@@ -85,6 +86,7 @@ def library() -> ctypes.CDLL:
     L.kpp_mech_name.argtypes = [ci]
     L.kpp_mech_info.argtypes = [cs, vp]
     L.kpp_mech_species.restype = cs
+    L.kpp_mech_initial.argtypes = [cs, vp]
     L.kpp_mech_species.argtypes = [cs, ci]
     L.kpp_mech_lu_pattern.argtypes = [cs, vp, vp, vp]
     L.kpp_check_params.argtypes = [vp, vp, vp, vp, ci]
@@ -97,6 +99,7 @@ def library() -> ctypes.CDLL:
     L.kpp_decomp_dev.argtypes = [cs, ci, ci, vp, vp, vp]
     L.kpp_solve_dev.argtypes = [cs, ci, ci, vp, vp, vp]
     L.kpp_rates_dev.argtypes = [cs, ci, ci, vp, vp, vp, vp, vp, vp]
+    L.kpp_rates_c_dev.argtypes = [cs, ci, ci, vp, vp, vp, vp, vp, vp, vp]
     L.kpp_ctx_create.argtypes = [ctypes.POINTER(vp), cs, ci, ci, ci]
     L.kpp_ctx_destroy.argtypes = [vp]
     L.kpp_set_state.argtypes = [vp, vp, vp, vp, vp]
@@ -119,6 +122,8 @@ class Mechanism:
     lu_nonzero: int
     rates_use_time: bool
     n_sun_rates: int
+    rates_use_state: bool
+    n_state_rates: int
     species: tuple
     stats: dict
 
@@ -134,14 +139,15 @@ def mechanisms() -> list[str]:
 
 def mechanism(name: str) -> Mechanism:
     L = library()
-    out = np.zeros(8, np.int32)
+    out = np.zeros(10, np.int32)
     r = L.kpp_mech_info(name.encode(), out.ctypes.data)
     if r != 0:
         raise KppError(f"unknown mechanism {name!r} (compiled: {mechanisms()})")
     sp = tuple(L.kpp_mech_species(name.encode(), k).decode() for k in range(out[2]))
     stats_path = GEN / f"{name}_stats.json"
     stats = json.loads(stats_path.read_text()) if stats_path.exists() else {}
-    return Mechanism(name, *map(int, out[:6]), bool(out[6]), int(out[7]), sp, stats)
+    return Mechanism(name, *map(int, out[:6]), bool(out[6]), int(out[7]), bool(out[8]),
+                     int(out[9]), sp, stats)
 
 
 def lu_pattern(name: str):
@@ -202,15 +208,15 @@ _default_ws = {}
 
 
 def workspace_bytes(mech: str, method: int, ncells: int, variant=PLAIN, nblocks=0,
-                    refill_min=0, lanes=0) -> int:
-    l = _Launch(variant, nblocks, refill_min, lanes)
+                    refill_min=0, lanes=0, lu_fp32=0) -> int:
+    l = _Launch(variant, nblocks, refill_min, lanes, lu_fp32)
     return int(library().kpp_workspace_bytes(mech.encode(), method, ctypes.byref(l), ncells))
 
 
 def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, RPAR=None,
                sun=None, sun_mode=SUN_FIXED, hstart=None, rconst=None, order=None,
                hfirst=None, ierr=None, istat=None, rstat=None, variant=PLAIN, nblocks=0, refill_min=0,
-               lanes=0, workspace=None, stream=None):
+               lanes=0, lu_fp32=0, fallbacks=None, workspace=None, stream=None):
     """Batched Rosenbrock on device tensors (float64 SoA, C is [NSPEC, ncells]).
 
     tstart/tend: per-cell tensors or floats.  atol/rtol: host sequences of
@@ -218,6 +224,9 @@ def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, R
     code 0 or Rosenbrock's parameter error; launches asynchronously on
     ``stream`` (torch's current stream by default).  ``hfirst`` (float64
     [ncells], optional) receives each cell's first accepted step size.
+    ``lu_fp32=1`` selects the mixed-precision integrator (FP32 LU, one FP64
+    refinement sweep per solve, per-cell FP64 fallback counted in the int32
+    [ncells] ``fallbacks``).
     """
     import torch
     L = library()
@@ -254,11 +263,12 @@ def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, R
     b.rstat = rstat.data_ptr()
     b.order = order.data_ptr() if order is not None else None
     b.hfirst = hfirst.data_ptr() if hfirst is not None else None
+    b.fallbacks = fallbacks.data_ptr() if fallbacks is not None else None
     IPAR = np.ascontiguousarray(IPAR, np.int32)
     RPAR = np.zeros(20) if RPAR is None else np.ascontiguousarray(RPAR, np.float64)
     atol = np.ascontiguousarray(np.atleast_1d(atol), np.float64)
     rtol = np.ascontiguousarray(np.atleast_1d(rtol), np.float64)
-    l = _Launch(variant, nblocks, refill_min, lanes)
+    l = _Launch(variant, nblocks, refill_min, lanes, lu_fp32)
     nbytes = L.kpp_workspace_bytes(mech.encode(), int(IPAR[3]), ctypes.byref(l), n)
     if workspace is None:
         workspace = _default_ws.setdefault(str(dev), Workspace(dev))
@@ -301,8 +311,14 @@ def solve(mech, JVS, X, stream=None):
     _comp(library().kpp_solve_dev, mech.encode(), n, n, _ptr(JVS), _ptr(X), _stream_ptr(stream))
 
 
-def rates(mech, temp, cfactor, sun, time, out, stream=None):
+def rates(mech, temp, cfactor, sun, time, out, stream=None, C=None):
+    """Rate constants per cell; C ([NSPEC, n]) is the state that rate code
+    reading concentrations sees (required for such mechanisms)."""
     n = temp.shape[0]
+    if C is not None:
+        _comp(library().kpp_rates_c_dev, mech.encode(), n, n, _ptr(temp), _ptr(cfactor),
+              _ptr(sun), _ptr(time), _ptr(C), _ptr(out), _stream_ptr(stream))
+        return
     _comp(library().kpp_rates_dev, mech.encode(), n, n, _ptr(temp), _ptr(cfactor), _ptr(sun),
           _ptr(time), _ptr(out), _stream_ptr(stream))
 

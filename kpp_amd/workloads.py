@@ -142,8 +142,36 @@ def saprc99_uneven(ncells=1048576, seed=3, cells=None, rtol=1e-3, atol=1.0,
     return _saprc99(ncells, seed, cells, True, rtol, atol, method)
 
 
+# MCM isoprene example (examples/mcm/driver_mcm.f90): air density M = 2.5E19
+# molecules/cm3, initial mixing ratios of O3, NO2, CH4 and C5H8 times M, all
+# other species from Initialize (zero).  The mechanism's rate code takes M as
+# CFACTOR*1e6 and O2, N2, H2O as the driver's fractions of M.
+MCM_M = 2.5e19
+MCM_INIT = {"O3": 3.0e-8, "NO2": 1.0e-10, "CH4": 1.8e-6, "C5H8": 1.0e-9}
+
+
+def mcm_mixed(ncells=16384, seed=5, cells=None, rtol=1e-3, atol=1.0, method=solver.RODAS3):
+    """MCM isoprene subset: T ~ U(260, 310) K, SUN ~ U(0, 1) taken as cos(zenith),
+    initial concentrations = the driver's values x 10^U(-1, 1), 1 h, Rodas3."""
+    cells = np.arange(ncells, dtype=np.int64) if cells is None else np.asarray(cells)
+    vals, cf0, st = mech_init("mcm")
+    nvar = st["NVAR"]
+    base = vals * cf0
+    for name, mix in MCM_INIT.items():
+        base[st["species"].index(name)] = mix * MCM_M
+    C = _scale_var(base, nvar, seed, cells)
+    n = len(cells)
+    T = 260.0 + 50.0 * uniform(seed, F_TEMP, cells)
+    sun = uniform(seed, F_SUN, cells)
+    tstart = np.zeros(n)
+    return Batch("mcm", method, C, T, np.full(n, MCM_M / 1e6), sun, solver.SUN_FIXED, tstart,
+                 tstart + 3600.0, np.array([atol]), np.array([rtol]), solver.default_ipar(method),
+                 np.zeros(20), cells, {"workload": "mcm_mixed"})
+
+
 WORKLOADS = {
     "small_strato_tod": small_strato_tod,
     "saprc99_day": saprc99_day,
     "saprc99_uneven": saprc99_uneven,
+    "mcm_mixed": mcm_mixed,
 }

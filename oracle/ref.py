@@ -31,6 +31,7 @@ def lib(mech: str, fast: bool = False) -> ctypes.CDLL:
     L.ros_ref_update_sun.argtypes = [cd]
     L.ros_ref_wlamch.restype = cd
     L.ros_ref_rconst.argtypes = [cd, cd, cd, cd, vp]
+    L.ros_ref_rconst_c.argtypes = [cd, cd, cd, cd, vp, vp]
     L.ros_ref_fun.argtypes = [vp, vp, vp, vp]
     L.ros_ref_jac_sp.argtypes = [vp, vp, vp, vp]
     L.ros_ref_decomp.argtypes = [vp]

@@ -65,6 +65,8 @@ typedef struct {
   int rates_given;
   ros_stats st;
   double Hacc1; /* step size of the first accepted step of the call (step log), 0 if none */
+  const double* Yacc; /* the accepted state (the integrator's Y): what rate code reading
+                         concentrations sees, as KPP's global C */
 } ros_cell;
 
 typedef struct {
@@ -144,7 +146,7 @@ static int KppDecomp(double* JVS) {
 static void update_rates(ros_cell* c, double T) {
   if (c->rates_given) return;
   if (c->sun_mode == 1) c->SUN = ros_ref_update_sun(T);
-  kpp_Update_RCONST(c->TEMP, c->SUN, c->CFACTOR, T, c->RCONST);
+  kpp_Update_RCONST(c->TEMP, c->SUN, c->CFACTOR, T, c->Yacc, c->FIX, c->RCONST);
 }
 static void FunTemplate(ros_cell* c, double T, const double* Y, double* Ydot) {
   update_rates(c, T);
@@ -376,6 +378,7 @@ static int RosenbrockIntegrator(ros_cell* c, double* Y, double Tstart, double Te
 
   T = Tstart;
   *Hexit = 0.0;
+  c->Yacc = Y;
   H = MIN(Hstart, Hmax);
   if (ABS(H) <= 10.0 * Roundoff) H = DeltaMin;
   Direction = (Tend >= Tstart) ? +1 : -1;
@@ -537,7 +540,13 @@ void ros_ref_lu_pattern(int* icol, int* crow, int* diag) {
 const char* ros_ref_species_name(int k) { return (k >= 0 && k < KPP_NSPEC) ? kpp_SPC_NAMES[k] : 0; }
 void ros_ref_initialize(double* C) { kpp_Initialize(C); }
 void ros_ref_rconst(double TEMP, double SUN, double CFACTOR, double TIME, double* RCONST) {
-  kpp_Update_RCONST(TEMP, SUN, CFACTOR, TIME, RCONST);
+  static const double zero[KPP_NSPEC] = {0};
+  kpp_Update_RCONST(TEMP, SUN, CFACTOR, TIME, zero, zero + KPP_NVAR, RCONST);
+}
+/* rate constants seen by a cell whose concentrations are C[NSPEC] */
+void ros_ref_rconst_c(double TEMP, double SUN, double CFACTOR, double TIME, const double* C,
+                      double* RCONST) {
+  kpp_Update_RCONST(TEMP, SUN, CFACTOR, TIME, C, C + KPP_NVAR, RCONST);
 }
 void ros_ref_fun(const double* V, const double* F, const double* RCT, double* Vdot) {
   kpp_Fun(V, F, RCT, Vdot);
