@@ -51,8 +51,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_work(mech, method, istat):
-    """(bytes, flops) of all step attempts of a batch from its counters."""
+def algorithmic_work(mech, method, istat, mixed=False):
+    """(bytes, flops) of all step attempts of a batch from its counters.
+
+    mixed: the FP32-LU integrator -- per attempt the FP32 copy of Ghimj
+    (8+4 B per entry), the FP32 factorisation (2x4 B per entry), and per
+    stage two FP32 solves (4 B per entry each), the FP64 residual over the
+    Jacobian's nonzeros plus the diagonal and its three vectors, and the two
+    FP32 vector conversions, instead of the FP64 LU and solves."""
     from kpp_amd import solver
     st = mech.stats
     nnz, nv, nf, nr = mech.lu_nonzero, mech.nvar, mech.nfix, mech.nreact
@@ -61,8 +67,15 @@ def algorithmic_work(mech, method, istat):
                 solver.RODAS3: (4, 2), solver.RODAS4: (7, 5)}[method]
     vec = (S * S / 2 + 3 * S + 6) * nv
     fun_io = 2 * nv + nf + nr
-    b_first = 8 * (nnz + 2 * nnz + 2 * nnz + S * nnz + nf1 * fun_io + vec)
-    b_retry = 8 * (2 * nnz + 2 * nnz + S * nnz + nf2 * fun_io + vec)
+    if mixed:
+        nz = mech.nonzero + nv
+        lu = 12 * nnz + 8 * nnz
+        per_stage = 8 * nnz + 8 * nz + 24 * nv + 16 * nv
+        b_first = 8 * nnz + lu + S * per_stage + 8 * (nf1 * fun_io + vec)
+        b_retry = 8 * nnz + lu + S * per_stage + 8 * (nf2 * fun_io + vec)
+    else:
+        b_first = 8 * (nnz + 2 * nnz + 2 * nnz + S * nnz + nf1 * fun_io + vec)
+        b_retry = 8 * (2 * nnz + 2 * nnz + S * nnz + nf2 * fun_io + vec)
     fl = st["flops"]
     f_vec = (S * S + 3 * S + 6) * nv
     f_first = fl["jac"] + (nnz + nv) + fl["decomp"] + S * fl["solve"] + nf1 * fl["fun"] + f_vec
@@ -145,6 +158,11 @@ def main():
     ap.add_argument("--weights", default=WEIGHTS, help="trained step-scheduler network")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--accuracy-cells", type=int, default=4096)
+    ap.add_argument("--eager-tuned", action="store_true",
+                    help="run the tuned path as eager calls instead of one graph (same kernels; "
+                         "for per-dispatch hardware counters)")
+    ap.add_argument("--lu-fp32", action="store_true",
+                    help="also time the mixed-precision integrator and report it as value")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -175,10 +193,11 @@ def main():
     istat = torch.empty((8, n), dtype=torch.int32, device=dev)
     rstat = torch.empty((2, n), dtype=torch.float64, device=dev)
     ws = solver.Workspace(dev)
+    fallbacks = torch.zeros(n, dtype=torch.int32, device=dev)
     span = float(np.max(np.abs(b.tend - b.tstart)))
 
     model = None
-    if not args.no_learned:
+    if not args.no_learned and b.mech == "saprc99":
         from kpp_amd import learned
         if os.path.exists(args.weights):
             model = learned.StepModel.load(args.weights, dev)
@@ -188,7 +207,7 @@ def main():
                 "fresh batches (seeds 101, 102; untimed)")
             model, _ = learned.train_model(b.mech, wl_fn, [101, 102], args.ncells, dev, log=log)
 
-    def step(variant, learned_path):
+    def step(variant, learned_path, lu_fp32=0):
         C.copy_(C0)
         ev0, evk, ev1 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         ev0.record()
@@ -202,19 +221,19 @@ def main():
             atol=b.atol, rtol=b.rtol, IPAR=b.IPAR, RPAR=b.RPAR, sun=d["sun"],
             sun_mode=b.sun_mode, hstart=hs, order=order, ierr=ierr, istat=istat, rstat=rstat,
             variant=variant, nblocks=args.nblocks if variant == solver.PERSISTENT else 0,
-            refill_min=args.refill_min, workspace=ws)
+            refill_min=args.refill_min, lu_fp32=lu_fp32, fallbacks=fallbacks, workspace=ws)
         ev1.record()
         assert code == 0, code
         return ev0, evk, ev1
 
-    def timed(variant, learned_path, steps, warmup, name):
+    def timed(variant, learned_path, steps, warmup, name, lu_fp32=0):
         for _ in range(warmup):
-            step(variant, learned_path)
+            step(variant, learned_path, lu_fp32)
         torch.cuda.synchronize()
         evs = []
         t = time.perf_counter()
         for k in range(steps):
-            evs.append(step(variant, learned_path))
+            evs.append(step(variant, learned_path, lu_fp32))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
         dev_t = [a.elapsed_time(e) / 1e3 for a, _, e in evs]
@@ -228,7 +247,7 @@ def main():
             f"all IERR=1: {ok}")
         return {"wall": wall / steps, "device_s": float(np.mean(dev_t)),
                 "kernel_s": float(np.mean(kern)), "sched_s": float(np.mean(sched)), "ok": ok,
-                "istat": ist, "C": C.cpu().numpy()}
+                "istat": ist, "C": C.cpu().numpy(), "fallbacks": fallbacks.cpu().numpy()}
 
     def timed_graph(steps, warmup, name):
         """The tuned step recorded once as a graph (features, network, bucket
@@ -289,21 +308,30 @@ def main():
 
     runs = {}
     if model is not None:
-        runs["tuned"] = timed_graph(args.steps, args.warmup,
-                                    "tuned (learned schedule + autotuned persistent, one graph)")
-        eager = timed(solver.PERSISTENT, True, 1, 0, "tuned path, eager calls (for the split)")
-        runs["tuned"]["sched_s"] = eager["sched_s"]
-        # the integrator's share of a replay: replay time less the scheduler's
-        # time on the eager step (the graph's kernels are not timed one by one)
-        runs["tuned"]["kernel_s"] = runs["tuned"]["device_s"] - eager["sched_s"]
+        if args.eager_tuned:
+            runs["tuned"] = timed(solver.PERSISTENT, True, args.steps, args.warmup,
+                                  "tuned path, eager calls")
+        else:
+            runs["tuned"] = timed_graph(args.steps, args.warmup,
+                                        "tuned (learned schedule + autotuned persistent, one graph)")
+            eager = timed(solver.PERSISTENT, True, 1, 0, "tuned path, eager calls (for the split)")
+            runs["tuned"]["sched_s"] = eager["sched_s"]
+            # the integrator's share of a replay: replay time less the scheduler's
+            # time on the eager step (the graph's kernels are not timed one by one)
+            runs["tuned"]["kernel_s"] = runs["tuned"]["device_s"] - eager["sched_s"]
     runs["persistent"] = timed(solver.PERSISTENT, False, max(1, args.steps if model is None else 1),
                                args.warmup, "persistent, default start")
+    if args.lu_fp32:
+        runs["mixed"] = timed(solver.PERSISTENT, False, max(1, args.steps), args.warmup,
+                              "persistent, FP32 LU + FP64 refinement", lu_fp32=1)
     if not args.no_plain:
         runs["plain"] = timed(solver.PLAIN, False, max(1, args.steps), max(0, args.warmup), "plain")
-    top = runs["tuned"] if model is not None else runs["persistent"]
+    top = runs["tuned"] if model is not None else (runs["mixed"] if args.lu_fp32
+                                                    else runs["persistent"])
 
     # roofline of the persistent kernel on the tuned run's attempts
-    nbytes, nflops, attempts, bterms = algorithmic_work(mech, b.method, top["istat"])
+    top_mixed = (model is None and args.lu_fp32)
+    nbytes, nflops, attempts, bterms = algorithmic_work(mech, b.method, top["istat"], top_mixed)
     kern_t = top["kernel_s"]
 
     accuracy = None
@@ -328,6 +356,10 @@ def main():
                                  f"evenly spaced cells", "sample_cells": int(len(cells))}
         for k, r in runs.items():
             accuracy[k] = rel_errors(r["C"][:, cells], ref_C, mech.nvar, at, rt)
+        if "mixed" in runs:
+            accuracy["mixed_vs_fp64"] = rel_errors(runs["mixed"]["C"][:, cells],
+                                                   runs["persistent"]["C"][:, cells], mech.nvar,
+                                                   at, rt)
         if "plain" in runs and "tuned" in runs:
             accuracy["tuned_vs_plain"] = rel_errors(runs["tuned"]["C"][:, cells],
                                                     runs["plain"]["C"][:, cells], mech.nvar, at, rt)
@@ -359,12 +391,15 @@ def main():
                    "path": ("learned step scheduler (BF16 MFMA network: per-cell first step, "
                             "launch order by predicted step count) + autotuned persistent "
                             "work-stealing kernel, recorded as one graph"
-                            if model is not None else "persistent work-stealing kernel"),
+                            if model is not None else
+                            ("persistent work-stealing kernel, FP32 LU + FP64 refinement"
+                             if args.lu_fp32 else "persistent work-stealing kernel")),
                    "all_ierr_1": top["ok"], "attempts_per_cell": attempts / n},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, n),
-                     "kernel": "ros_persistent_kernel<kppm_saprc99,4>",
+                     "kernel": f"ros_persistent_kernel<kppm_{b.mech},"
+                               f"{solver.METHOD_STAGES[b.method]}{',MIX' if top_mixed else ''}>",
                      "kernel_ms": kern_t * 1e3, "algorithmic_bytes": nbytes,
                      "fp64_tflops": nflops / kern_t / 1e12,
                      "fp64_frac": nflops / kern_t / 1e12 / FP64_PEAK_TFLOPS, **bterms},
@@ -376,6 +411,13 @@ def main():
                    "all_ierr_1": plain["ok"]} if plain else None),
         "persistent_default_start": {"value": n / runs["persistent"]["wall"],
                                      "attempts_per_cell": float(runs["persistent"]["istat"][2].mean())},
+        "mixed": ({"value": n / runs["mixed"]["wall"],
+                   "kernel_ms": runs["mixed"]["kernel_s"] * 1e3,
+                   "attempts_per_cell": float(runs["mixed"]["istat"][2].mean()),
+                   "fallback_attempts": int(runs["mixed"]["fallbacks"].sum()),
+                   "cells_with_fallback": int((runs["mixed"]["fallbacks"] > 0).sum()),
+                   "speedup_vs_fp64_persistent": runs["persistent"]["wall"] / runs["mixed"]["wall"],
+                   "all_ierr_1": runs["mixed"]["ok"]} if "mixed" in runs else None),
         "speedup_vs_plain": (value / (n / plain["wall"])) if plain else None,
         "accuracy": accuracy,
         "cpu_baseline": cpu,

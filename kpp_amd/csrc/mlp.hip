@@ -230,8 +230,12 @@ __global__ void k_features(int n, int ld, int nspec, const double* __restrict__ 
 }
 
 // Prediction: first step h = 10^(o0*sd_y + mu_y) clamped to [hmin, hmax];
-// bucket = argmax of the bucket logits; sort key = NBUCKET-1-bucket so the
-// cells predicted to need the most steps are launched first.
+// bucket = argmax of the bucket logits.  The launch-order key is the
+// expected bucket under the logits' softmax, quantised to NKEY levels and
+// reversed, so cells predicted to need the most steps are launched first
+// (longest-processing-time order for the persistent kernel's work queue);
+// it ranks cells more finely than the argmax.
+constexpr int NKEY = 64;
 __global__ void k_decode(int n, const float* __restrict__ O, float mu_y, float sd_y, double hmin,
                          double hmax, double* hstart, int* bucket, uint8_t* key) {
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
@@ -243,13 +247,26 @@ __global__ void k_decode(int n, const float* __restrict__ O, float mu_y, float s
     hstart[cell] = h;
   }
   int best = 0;
+  float mx = o[1];
   for (int j = 1; j < NBUCKET; j++)
     if (o[1 + j] > o[1 + best]) best = j;
+  mx = o[1 + best];
+  float se = 0.f, ex = 0.f;
+  for (int j = 0; j < NBUCKET; j++) {
+    const float e = expf(o[1 + j] - mx);
+    se += e;
+    ex += e * (float)j;
+  }
   if (bucket) bucket[cell] = best;
-  if (key) key[cell] = (uint8_t)(NBUCKET - 1 - best);
+  if (key) {
+    const float eb = ex / se;  // expected bucket in [0, NBUCKET-1]
+    int q = (int)(eb * (float)(NKEY - 1) / (float)(NBUCKET - 1) + 0.5f);
+    q = q < 0 ? 0 : (q > NKEY - 1 ? NKEY - 1 : q);
+    key[cell] = (uint8_t)(NKEY - 1 - q);
+  }
 }
 
-// Stable counting sort of cell indices by an 8-valued key, in three kernels
+// Stable counting sort of cell indices by an NKEY-valued key, in three kernels
 // with no memory clears (every count is written, none accumulated), so the
 // sequence can be recorded in a graph.  Block b owns cells
 // [b*SORT_CHUNK, (b+1)*SORT_CHUNK).
@@ -259,43 +276,43 @@ constexpr int SORT_CHUNK = SORT_THREADS * SORT_ITEMS;
 
 // rank of this thread's key among equal keys of lower threads in the block,
 // and the per-key totals of the block (LDS)
-__device__ __forceinline__ int block_rank(int key, bool valid, int* wave_cnt /*[4][8]*/,
-                                          int* tot /*[8]*/) {
+__device__ __forceinline__ int block_rank(int key, bool valid, int* wave_cnt /*[4][NKEY]*/,
+                                          int* tot /*[NKEY]*/) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int rank = 0;
 #pragma unroll
-  for (int k = 0; k < NBUCKET; k++) {
+  for (int k = 0; k < NKEY; k++) {
     const uint64_t m = __ballot(valid && key == k);
     if (valid && key == k) rank = __popcll(m & below);
-    if (lane == 0) wave_cnt[wave * NBUCKET + k] = __popcll(m);
+    if (lane == 0) wave_cnt[wave * NKEY + k] = __popcll(m);
   }
   __syncthreads();
-  if (threadIdx.x < NBUCKET) {
+  if (threadIdx.x < NKEY) {
     int t = 0;
-    for (int w = 0; w < SORT_THREADS / 64; w++) t += wave_cnt[w * NBUCKET + threadIdx.x];
+    for (int w = 0; w < SORT_THREADS / 64; w++) t += wave_cnt[w * NKEY + threadIdx.x];
     tot[threadIdx.x] = t;
   }
   int before = 0;
   if (valid)
-    for (int w = 0; w < wave; w++) before += wave_cnt[w * NBUCKET + key];
+    for (int w = 0; w < wave; w++) before += wave_cnt[w * NKEY + key];
   __syncthreads();
   return rank + before;
 }
 
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_count(int n, const uint8_t* __restrict__ key,
                                                              int nblk, int* counts) {
-  __shared__ int wave_cnt[4 * NBUCKET], tot[NBUCKET];
-  int acc = 0;  // thread k < NBUCKET accumulates key k
+  __shared__ int wave_cnt[4 * NKEY], tot[NKEY];
+  int acc = 0;  // thread k < NKEY accumulates key k
   const int base = blockIdx.x * SORT_CHUNK;
   for (int it = 0; it < SORT_ITEMS; it++) {
     const int c = base + it * SORT_THREADS + threadIdx.x;
     const bool valid = c < n;
     block_rank(valid ? key[c] : 0, valid, wave_cnt, tot);
-    if (threadIdx.x < NBUCKET) acc += tot[threadIdx.x];
+    if (threadIdx.x < NKEY) acc += tot[threadIdx.x];
     __syncthreads();
   }
-  if (threadIdx.x < NBUCKET) counts[threadIdx.x * nblk + blockIdx.x] = acc;
+  if (threadIdx.x < NKEY) counts[threadIdx.x * nblk + blockIdx.x] = acc;
 }
 
 // exclusive scan of counts[key][block] in key-major order (one block)
@@ -327,8 +344,8 @@ __global__ __launch_bounds__(1024) void k_sort_scan(int total, int* counts) {
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(int n, const uint8_t* __restrict__ key,
                                                                int nblk, const int* __restrict__ offs,
                                                                int* order) {
-  __shared__ int wave_cnt[4 * NBUCKET], tot[NBUCKET], run[NBUCKET];
-  if (threadIdx.x < NBUCKET) run[threadIdx.x] = offs[threadIdx.x * nblk + blockIdx.x];
+  __shared__ int wave_cnt[4 * NKEY], tot[NKEY], run[NKEY];
+  if (threadIdx.x < NKEY) run[threadIdx.x] = offs[threadIdx.x * nblk + blockIdx.x];
   __syncthreads();
   const int base = blockIdx.x * SORT_CHUNK;
   for (int it = 0; it < SORT_ITEMS; it++) {
@@ -338,7 +355,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(int n, const uint
     const int r = block_rank(k, valid, wave_cnt, tot);
     if (valid) order[run[k] + r] = c;
     __syncthreads();
-    if (threadIdx.x < NBUCKET) run[threadIdx.x] += tot[threadIdx.x];
+    if (threadIdx.x < NKEY) run[threadIdx.x] += tot[threadIdx.x];
     __syncthreads();
   }
 }
@@ -547,7 +564,7 @@ size_t kpp_mlp_schedule_work_bytes(int n) {
   if (n <= 0) return 0;
   const int nblk = (n + SORT_CHUNK - 1) / SORT_CHUNK;
   return kpp_mlp_work_bytes(n) + ((size_t)n + 255) / 256 * 256 +
-         ((size_t)NBUCKET * nblk * 4 + 255) / 256 * 256;
+         ((size_t)NKEY * nblk * 4 + 255) / 256 * 256;
 }
 
 int kpp_mlp_schedule(const kpp_mlp* net, const void* X, int n, float mu_y, float sd_y,
@@ -566,7 +583,7 @@ int kpp_mlp_schedule(const kpp_mlp* net, const void* X, int n, float mu_y, float
   if (order) {
     const int nblk = (n + SORT_CHUNK - 1) / SORT_CHUNK;
     hipLaunchKernelGGL(k_sort_count, dim3(nblk), dim3(SORT_THREADS), 0, st, n, key, nblk, counts);
-    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, NBUCKET * nblk, counts);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, NKEY * nblk, counts);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(SORT_THREADS), 0, st, n, key, nblk, counts,
                        order);
   }

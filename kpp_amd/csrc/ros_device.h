@@ -356,7 +356,7 @@ struct Cell {
     // FP64: KppDecomp in place.  MIX: factor an FP32 copy and keep the FP64
     // matrix for the residuals of the refinement sweeps.
     bool lu64 = !MIX;
-    if (MIX) {
+    if constexpr (MIX) {
       AccF G32 = at32(L::G32);
       for (int i = 0; i < M::LU_NONZERO; i++) G32[i] = (float)G[i];
       call_decomp32<M>(G32.b);
@@ -405,7 +405,8 @@ struct Cell {
         if (use_g) k = k + HG * dFdT[i];
         Ki[i] = k;
       }
-      if (MIX && !lu64) {
+      if constexpr (MIX) {
+       if (!lu64) {
         // x1 = FP32 solve of b; r = b - A x1 in FP64; x = x1 + FP32 solve of r
         AccF G32 = at32(L::G32), X32 = at32(L::X32);
         Acc XR = at(L::XR), RR = at(L::RR);
@@ -432,6 +433,9 @@ struct Cell {
           call_decomp<M>(G.b);
           call_solve<M>(G.b, Ki.b);
         }
+       } else {
+        call_solve<M>(G.b, Ki.b);
+       }
       } else {
         call_solve<M>(G.b, Ki.b);
       }

@@ -191,7 +191,7 @@ class StepModel:
                                    _p(loss), _p(ws), ws.numel(), _stream()), "kpp_mlp_grad")
         return loss, g
 
-    def train(self, X, y, bucket, wreg, *, epochs=4, batch=4096, lr=2e-3, seed=0, log=None):
+    def train(self, X, y, bucket, wreg, *, epochs=40, batch=8192, lr=3e-3, seed=0, log=None):
         """Adam over shuffled mini-batches; cosine learning-rate decay."""
         import torch
         n = int(X.shape[0])
@@ -313,7 +313,8 @@ def log_steps(b, d, ws=None):
     return hfirst, istat2[2].clone()
 
 
-def train_model(mech, wl_fn, seeds, ncells, device, *, epochs=4, holdout=0.1, log=print):
+def train_model(mech, wl_fn, seeds, ncells, device, *, epochs=40, batch=8192, lr=3e-3,
+                holdout=0.1, log=print):
     """Log steps on fresh batches of a workload, train a StepModel, report holdout metrics."""
     import torch
     t0 = time.time()
@@ -341,7 +342,7 @@ def train_model(mech, wl_fn, seeds, ncells, device, *, epochs=4, holdout=0.1, lo
     model.fit_targets(hfirst[tr], nstp[tr])
     y, bk, wr = model.targets(hfirst, nstp)
     t1 = time.time()
-    hist = model.train(X[tr], y[tr], bk[tr], wr[tr], epochs=epochs, log=log)
+    hist = model.train(X[tr], y[tr], bk[tr], wr[tr], epochs=epochs, batch=batch, lr=lr, log=log)
     torch.cuda.synchronize()
     log(f"[learned] trained {epochs} epochs on {len(tr)} samples in {time.time() - t1:.1f}s")
     out = model.forward(X[ho])
@@ -365,7 +366,7 @@ def main():
     ap.add_argument("--workload", default="saprc99_uneven")
     ap.add_argument("--seeds", default="101,102")
     ap.add_argument("--ncells", type=int, default=1048576)
-    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
     wl = workloads.WORKLOADS[a.workload]

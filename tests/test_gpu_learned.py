@@ -133,11 +133,19 @@ def test_schedule_outputs(gpu):
     assert np.array_equal(b, O[:, 1:9].argmax(1))
     o = order.cpu().numpy()
     assert np.array_equal(np.sort(o), np.arange(n))
-    bo = b[o]
-    assert np.all(np.diff(bo) <= 0)  # most steps first
-    for k in range(8):  # stable within a bucket
-        sel = o[bo == k]
-        assert np.all(np.diff(sel) > 0)
+    # launch order: expected bucket under the softmax, most steps first, on 64
+    # levels (recomputed here in float64: a level can differ by one where the
+    # device's FP32 softmax rounds the other way)
+    lg = O[:, 1:9]
+    p = np.exp(lg - lg.max(1, keepdims=True))
+    eb = (p * np.arange(8)).sum(1) / p.sum(1)
+    key = 63 - np.clip(np.floor(eb * 9.0 + 0.5), 0, 63).astype(int)
+    ko = key[o]
+    assert np.all(np.diff(ko) >= -1)
+    assert np.mean(np.diff(ko) >= 0) > 0.99
+    # stable: within one key, cells keep their index order
+    same = np.diff(ko) == 0
+    assert np.mean(np.diff(o)[same] > 0) > 0.99
 
 
 def test_features_match_oracle(gpu):
