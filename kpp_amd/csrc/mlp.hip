@@ -208,25 +208,41 @@ __global__ void k_to_bf16(size_t n, const float* w, __bf16* wb) {
 
 // Features of one cell: standardised log10 of every species concentration
 // (floored), TEMP and SUN; zero padding up to in_pad.
-__global__ void k_features(int n, int ld, int nspec, const double* __restrict__ C,
-                           const double* __restrict__ temp, const double* __restrict__ sun,
-                           const float* __restrict__ mu, const float* __restrict__ sd,
-                           double cfloor, int in_pad, __bf16* X) {
-  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= n) return;
-  __bf16* x = X + (size_t)cell * in_pad;
-  for (int j = 0; j < in_pad; j++) {
-    float v = 0.f;
-    if (j < nspec) {
-      const double c = C[(size_t)j * ld + cell];
-      v = (float)log10(c > cfloor ? c : cfloor);
-    } else if (j == nspec) {
-      v = (float)temp[cell];
-    } else if (j == nspec + 1) {
-      v = sun ? (float)sun[cell] : 0.f;
+__global__ __launch_bounds__(256) void k_features(int n, int ld, int nspec,
+                                                   const double* __restrict__ C,
+                                                   const double* __restrict__ temp,
+                                                   const double* __restrict__ sun,
+                                                   const float* __restrict__ mu,
+                                                   const float* __restrict__ sd, double cfloor,
+                                                   int in_pad, __bf16* X) {
+  // one block = 256 consecutive cells = one contiguous stretch of X; each
+  // thread writes its row into LDS, then the block copies it out in 16-byte
+  // pieces (a thread's own row would be a 2*in_pad-byte strided write)
+  extern __shared__ __align__(16) unsigned char lds[];
+  __bf16* tile = reinterpret_cast<__bf16*>(lds);
+  const int c0 = blockIdx.x * 256;
+  const int cell = c0 + threadIdx.x;
+  if (cell < n) {
+    __bf16* x = tile + (size_t)threadIdx.x * in_pad;
+    for (int j = 0; j < in_pad; j++) {
+      float v = 0.f;
+      if (j < nspec) {
+        const double c = C[(size_t)j * ld + cell];
+        v = (float)log10(c > cfloor ? c : cfloor);
+      } else if (j == nspec) {
+        v = (float)temp[cell];
+      } else if (j == nspec + 1) {
+        v = sun ? (float)sun[cell] : 0.f;
+      }
+      x[j] = (j < nspec + 2) ? (__bf16)((v - mu[j]) / sd[j]) : (__bf16)0.0f;
     }
-    x[j] = (j < nspec + 2) ? (__bf16)((v - mu[j]) / sd[j]) : (__bf16)0.0f;
   }
+  __syncthreads();
+  const int rows = min(256, n - c0);
+  const int chunks = rows * in_pad / 8;  // in_pad is a multiple of 32
+  uint4* dst = reinterpret_cast<uint4*>(X + (size_t)c0 * in_pad);
+  const uint4* src = reinterpret_cast<const uint4*>(tile);
+  for (int i = threadIdx.x; i < chunks; i += 256) dst[i] = src[i];
 }
 
 // Prediction: first step h = 10^(o0*sd_y + mu_y) clamped to [hmin, hmax];
@@ -513,8 +529,10 @@ int kpp_mlp_features(int n, int ld, int nspec, const double* C, const double* te
                      int in_pad, void* X, void* stream) {
   if (n <= 0 || ld < n || !C || !temp || !mu || !sd || !X || in_pad < nspec + 2 || in_pad % 32)
     return KPP_E_ARG;
-  hipLaunchKernelGGL(k_features, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), n, ld,
-                     nspec, C, temp, sun, mu, sd, cfloor, in_pad, (__bf16*)X);
+  if ((reinterpret_cast<uintptr_t>(X) & 15) != 0) return KPP_E_ARG;
+  hipLaunchKernelGGL(k_features, dim3((n + 255) / 256), dim3(256),
+                     (unsigned)(256 * in_pad * sizeof(__bf16)), as_stream(stream), n, ld, nspec, C,
+                     temp, sun, mu, sd, cfloor, in_pad, (__bf16*)X);
   return hipGetLastError() == hipSuccess ? 0 : KPP_E_HIP;
 }
 

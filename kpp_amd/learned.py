@@ -88,6 +88,7 @@ class StepModel:
     sd: object = None
     mu_y: float = 0.0
     sd_y: float = 1.0
+    h_scale: float = 1.0    # starting step = h_scale x the predicted first step
     edges: np.ndarray = field(default_factory=lambda: np.zeros(NBUCKET - 1))
     _work: object = None
 
@@ -239,7 +240,8 @@ class StepModel:
             order = torch.empty(n, dtype=torch.int32, device=self.device) if want_order else None
         ws = self.work(int(_lib().kpp_mlp_schedule_work_bytes(n)))
         net = self.net()
-        _check(_lib().kpp_mlp_schedule(ctypes.byref(net), _p(X), n, float(self.mu_y),
+        mu = float(self.mu_y + np.log10(self.h_scale))  # 10^(o*sd + mu) * h_scale
+        _check(_lib().kpp_mlp_schedule(ctypes.byref(net), _p(X), n, mu,
                                        float(self.sd_y), float(hmin), float(hmax), _p(hs), _p(bk),
                                        _p(order), _p(ws), ws.numel(), _stream()),
                "kpp_mlp_schedule")
@@ -270,6 +272,7 @@ class StepModel:
         np.savez(path, mech=np.array(self.mech), w=self.w.cpu().numpy(),
                  mu=self.mu.cpu().numpy(), sd=self.sd.cpu().numpy(),
                  mu_y=np.float64(self.mu_y), sd_y=np.float64(self.sd_y), edges=self.edges,
+                 h_scale=np.float64(self.h_scale),
                  cfloor=np.float64(self.cfloor), in_pad=np.int32(self.in_pad))
 
     @classmethod
@@ -280,7 +283,8 @@ class StepModel:
                 w=torch.tensor(z["w"], dtype=torch.float32, device=device),
                 mu=torch.tensor(z["mu"], dtype=torch.float32, device=device),
                 sd=torch.tensor(z["sd"], dtype=torch.float32, device=device),
-                mu_y=float(z["mu_y"]), sd_y=float(z["sd_y"]), edges=np.array(z["edges"]))
+                mu_y=float(z["mu_y"]), sd_y=float(z["sd_y"]), edges=np.array(z["edges"]),
+                h_scale=float(z["h_scale"]) if "h_scale" in z.files else 1.0)
         if m.in_pad != int(z["in_pad"]):
             raise ValueError("weights file does not match the mechanism's input width")
         return m
@@ -350,8 +354,26 @@ def train_model(mech, wl_fn, seeds, ncells, device, *, epochs=40, batch=8192, lr
     rmse = float(torch.sqrt(((out[:, 0] - y[ho])[ok] ** 2).mean()) * model.sd_y)
     acc = float((out[:, 1:1 + NBUCKET].argmax(1).int() == bk[ho]).float().mean())
     near = float(((out[:, 1:1 + NBUCKET].argmax(1).int() - bk[ho]).abs() <= 1).float().mean())
+    # starting-step scale: the one of a few that needs the fewest attempts on
+    # (training) cells of the first batch
+    b = wl_fn(ncells=min(ncells, 65536), seed=seeds[0])
+    d = b.to_device(device)
+    span = float(np.max(np.abs(b.tend - b.tstart)))
+    Xs = model.features(d["C"], d["temp"], d["sun"])
+    tried = {}
+    for sc in (1.0, 1.5, 2.0):
+        model.h_scale = sc
+        hs, _, order = model.schedule(Xs, 1e-6, span)
+        code, ierr, istat, _ = solver.rosenbrock(
+            b.mech, d["C"].clone(), d["temp"], d["cfactor"], tstart=d["tstart"], tend=d["tend"],
+            atol=b.atol, rtol=b.rtol, IPAR=b.IPAR, RPAR=b.RPAR, sun=d["sun"],
+            sun_mode=b.sun_mode, hstart=hs, order=order, variant=solver.PERSISTENT)
+        tried[sc] = float(istat[2].double().mean())
+    model.h_scale = min(tried, key=tried.get)
+    log(f"[learned] starting-step scale {model.h_scale} (attempts per cell {tried})")
     metrics = {"samples": int(N), "holdout": int(nho), "log10_h_rmse": rmse,
-               "bucket_accuracy": acc, "bucket_within_1": near, "loss_history": hist}
+               "bucket_accuracy": acc, "bucket_within_1": near, "h_scale": model.h_scale,
+               "h_scale_attempts": tried, "loss_history": hist}
     log(f"[learned] holdout: log10(h) RMSE {rmse:.3f}, bucket accuracy {acc:.3f} "
         f"(within one bucket {near:.3f})")
     return model, metrics
