@@ -56,10 +56,17 @@ HIP_FLAGS = [
     # packed FP32 instructions stay off in every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-function", "-Wno-unused-variable",
-    # register budget for >= 4 resident wavefronts per SIMD: the integrator
-    # is latency bound and twice as fast at 4 waves/SIMD as at 1-2
-    "-DKPPHIP_WAVES_PER_EU=4",
 ]
+
+# Register budget of each mechanism's integrator kernels, as the number of
+# wavefronts per SIMD the allocator must leave room for (512/N VGPRs per
+# lane, shared by the out-of-line model routines).  Measured on MI355X
+# (persistent kernel, index order): saprc99 on 1,048,576 uneven cells takes
+# 1.85 s at 4 waves/SIMD (128 VGPRs, 2.6 KB of spills per lane), 1.71 s at 2
+# (256 VGPRs) and 2.07 s at 1; MCM on 16,384 cells 2.35 s at 4 (its 16,384
+# cells fill one wavefront per SIMD at most), 2.03 s at 2 and 1.88 s at 1
+# (VGPRs plus AGPRs for spills).  small_strato fits in registers at 4.
+WAVES_PER_EU = {"small_strato": 4, "saprc99": 2, "mcm": 1}
 
 
 def kpp_home() -> Path:
@@ -111,9 +118,14 @@ def generate(mechs=None) -> list[str]:
     return done
 
 
+def _mech_of(src: Path) -> str:
+    return src.stem.replace("kpphip_", "", 1).replace("_mixed", "").replace("_comp", "")
+
+
 def _hipcc_obj(src: Path, out: Path):
-    run([HIPCC, *HIP_FLAGS, "-I", str(CSRC), "-I", str(GEN), "-I", str(REPO / "include"),
-         "-c", "-x", "hip", str(src), "-o", str(out)])
+    waves = WAVES_PER_EU.get(_mech_of(src), 4)
+    run([HIPCC, *HIP_FLAGS, f"-DKPPHIP_WAVES_PER_EU={waves}", "-I", str(CSRC), "-I", str(GEN),
+         "-I", str(REPO / "include"), "-c", "-x", "hip", str(src), "-o", str(out)])
     return out
 
 
@@ -126,8 +138,7 @@ def build_lib(mechs: list[str], jobs: int = 8) -> Path:
             [CSRC / "kpphip_api.cpp", CSRC / "mlp.hip"])
     # largest mechanism first, so the slowest translation units start at once
     def weight(src):
-        h = GEN / (src.stem.replace("kpphip_", "", 1).replace("_mixed", "").replace("_comp", "")
-                   + "_hip.h")
+        h = GEN / (_mech_of(src) + "_hip.h")
         return -(h.stat().st_size if h.exists() else 0)
     srcs.sort(key=weight)
     objs = [OBJ / (s.stem + ".o") for s in srcs]

@@ -1816,6 +1816,44 @@ void emit_hip(Model& md, const std::string& path) {
                         std::to_string(k + 1) + ";\n";
       std::map<int, int> pos;  // column -> position in row k
       for (int kk = S.lu_crow[k]; kk < S.lu_crow[k + 1]; kk++) pos[S.lu_icol[kk]] = kk;
+      if (!checked) {
+        // Row k held in locals: every entry the elimination touches is loaded
+        // first and stored once at the end, so no store of row k sits between
+        // the loads of the pivot rows it reads (earlier rows only) and they
+        // can all be in flight together.  Same operations, in the same order
+        // per entry, as the statement form below.
+        std::vector<int> touched;
+        std::string ops;
+        for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
+          int j = S.lu_icol[kk];
+          touched.push_back(kk);
+          ops += "    w" + std::to_string(kk) + " = w" + std::to_string(kk) + "/G[" +
+                 std::to_string(S.lu_diag[j]) + "];";
+          n_ops++;
+          for (int jj = S.lu_diag[j] + 1; jj < S.lu_crow[j + 1]; jj++) {
+            auto it = pos.find(S.lu_icol[jj]);
+            if (it == pos.end()) die("LU pattern not closed under elimination");
+            const std::string w = "w" + std::to_string(it->second);
+            touched.push_back(it->second);
+            ops += " " + w + " = " + w + " - w" + std::to_string(kk) + "*G[" + std::to_string(jj) +
+                   "];";
+            n_ops += 2;
+          }
+          ops += "\n";
+        }
+        if (!touched.empty()) {
+          std::sort(touched.begin(), touched.end());
+          touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+          t += "  {\n   ";
+          for (int x : touched)
+            t += " typename TG::value_type w" + std::to_string(x) + " = G[" + std::to_string(x) + "];";
+          t += "\n" + ops + "   ";
+          for (int x : touched) t += " G[" + std::to_string(x) + "] = w" + std::to_string(x) + ";";
+          t += "\n  }\n";
+        }
+        st.push_back({t, n_ops});
+        continue;
+      }
       for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
         int j = S.lu_icol[kk];
         t += "  { const auto l = G[" + std::to_string(kk) + "]/G[" + std::to_string(S.lu_diag[j]) +

@@ -10,6 +10,7 @@ namespace kpphip {
 
 // Accessor over the caller's SoA arrays: element i of cell c at b[i*ld + c].
 struct Strided {
+  using value_type = double;
   double* __restrict__ b;
   size_t ld;
   __device__ __forceinline__ double& operator[](int i) const { return b[(size_t)i * ld]; }

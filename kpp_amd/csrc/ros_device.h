@@ -28,7 +28,7 @@
 
 #include "kpphip_params.h"
 
-// Build knobs (set by the autotuned build, kpp_amd/build.py):
+// Build knobs (set per mechanism by kpp_amd/build.py):
 //  KPPHIP_WAVES_PER_EU  minimum resident wavefronts per SIMD the register
 //                       allocator must allow (0: compiler default)
 //  KPPHIP_MODEL_INLINE  1: inline the model routines into the kernels;
@@ -51,10 +51,14 @@
 #else
 #define KPPHIP_KERNEL_ATTR
 #endif
+// The out-of-line routines must stay internal (no `used`): the compiler
+// then derives their register budget from their callers' launch bounds and
+// waves-per-EU; an externally visible routine is compiled for the default
+// 1024-thread workgroup, i.e. capped at 128 VGPRs whatever the kernel allows.
 #if KPPHIP_MODEL_INLINE
 #define KPPHIP_MODEL_CALL __device__ __forceinline__
 #else
-#define KPPHIP_MODEL_CALL __device__ __noinline__ __attribute__((used))
+#define KPPHIP_MODEL_CALL __device__ __noinline__
 #endif
 
 namespace kpphip {
@@ -68,13 +72,16 @@ constexpr int TILE = 64;
 // return in order and can be kept in flight back to back.
 using gdouble = __attribute__((address_space(1))) double;
 
-// Element accessor for a lane inside a 64-lane tile.
+// Element accessor for a lane inside a 64-lane tile.  value_type is the
+// arithmetic type of the emitted routines' locals.
 struct Acc {
+  using value_type = double;
   gdouble* __restrict__ b;
   __device__ __forceinline__ gdouble& operator[](int i) const { return b[i * TILE]; }
 };
 using gfloat = __attribute__((address_space(1))) float;
 struct AccF {
+  using value_type = float;
   gfloat* __restrict__ b;
   __device__ __forceinline__ gfloat& operator[](int i) const { return b[i * TILE]; }
 };
