@@ -233,12 +233,15 @@ int kpp_mlp_train_step(const kpp_mlp* net, const void* X, const float* y, const 
                        const float* wreg, int n, float* grad, float* m1, float* m2, int step,
                        float lr, float* loss, void* work, size_t work_bytes, void* stream);
 /* prediction: hstart[n] = 10^(o0*sd_y+mu_y) clamped to [hmin,hmax], bucket[n],
- * order[n] = cells sorted by predicted bucket, most steps first (stable);
+ * order[n] = cells sorted by expected step count, most steps first (stable):
+ * sum_j softmax(o[1..8])_j * centres[j] on 64 levels between centres[0] and
+ * centres[7] (host array of 8 increasing floats, the mean step count of each
+ * bucket), or the expected bucket index if centres is NULL;
  * any output may be NULL */
 size_t kpp_mlp_schedule_work_bytes(int n);
 int kpp_mlp_schedule(const kpp_mlp* net, const void* X, int n, float mu_y, float sd_y,
-                     double hmin, double hmax, double* hstart, int* bucket, int* order,
-                     void* work, size_t work_bytes, void* stream);
+                     double hmin, double hmax, const float* centres, double* hstart, int* bucket,
+                     int* order, void* work, size_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }

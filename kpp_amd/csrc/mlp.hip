@@ -247,13 +247,21 @@ __global__ __launch_bounds__(256) void k_features(int n, int ld, int nspec,
 
 // Prediction: first step h = 10^(o0*sd_y + mu_y) clamped to [hmin, hmax];
 // bucket = argmax of the bucket logits.  The launch-order key is the
-// expected bucket under the logits' softmax, quantised to NKEY levels and
-// reversed, so cells predicted to need the most steps are launched first
-// (longest-processing-time order for the persistent kernel's work queue);
-// it ranks cells more finely than the argmax.
+// expected step count under the logits' softmax -- sum_j p_j * centre_j, with
+// centre_j the mean step count of bucket j in the training data -- mapped
+// linearly from [centre_0, centre_last] onto NKEY levels and reversed, so
+// cells predicted to need the most steps are launched first
+// (longest-processing-time order for the persistent kernel's work queue).
+// Without centres the expected bucket index is used.  Expected counts rank
+// cells better than expected indices because the buckets are quantiles, not
+// equal widths in step count.
 constexpr int NKEY = 64;
+struct Centres {
+  float c[NBUCKET];
+  int use;
+};
 __global__ void k_decode(int n, const float* __restrict__ O, float mu_y, float sd_y, double hmin,
-                         double hmax, double* hstart, int* bucket, uint8_t* key) {
+                         double hmax, Centres cen, double* hstart, int* bucket, uint8_t* key) {
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= n) return;
   const float* o = O + (size_t)cell * NOUT;
@@ -271,12 +279,15 @@ __global__ void k_decode(int n, const float* __restrict__ O, float mu_y, float s
   for (int j = 0; j < NBUCKET; j++) {
     const float e = expf(o[1 + j] - mx);
     se += e;
-    ex += e * (float)j;
+    ex += e * (cen.use ? cen.c[j] : (float)j);
   }
   if (bucket) bucket[cell] = best;
   if (key) {
-    const float eb = ex / se;  // expected bucket in [0, NBUCKET-1]
-    int q = (int)(eb * (float)(NKEY - 1) / (float)(NBUCKET - 1) + 0.5f);
+    // expected count (or bucket index), position in [lo, hi] -> [0, NKEY-1]
+    const float lo = cen.use ? cen.c[0] : 0.f;
+    const float hi = cen.use ? cen.c[NBUCKET - 1] : (float)(NBUCKET - 1);
+    const float eb = ex / se;
+    int q = (int)((eb - lo) * (float)(NKEY - 1) / (hi - lo) + 0.5f);
     q = q < 0 ? 0 : (q > NKEY - 1 ? NKEY - 1 : q);
     key[cell] = (uint8_t)(NKEY - 1 - q);
   }
@@ -586,9 +597,18 @@ size_t kpp_mlp_schedule_work_bytes(int n) {
 }
 
 int kpp_mlp_schedule(const kpp_mlp* net, const void* X, int n, float mu_y, float sd_y,
-                     double hmin, double hmax, double* hstart, int* bucket, int* order,
-                     void* work, size_t work_bytes, void* stream) {
+                     double hmin, double hmax, const float* centres, double* hstart, int* bucket,
+                     int* order, void* work, size_t work_bytes, void* stream) {
   if (!net_ok(net) || !X || n <= 0 || !work || !(hmin > 0) || !(hmax >= hmin)) return KPP_E_ARG;
+  Centres cen{};
+  if (centres) {
+    for (int j = 0; j < NBUCKET; j++) {
+      cen.c[j] = centres[j];
+      if (!(centres[j] == centres[j])) return KPP_E_ARG;  // NaN
+      if (j > 0 && !(centres[j] > centres[j - 1])) return KPP_E_ARG;  // increasing
+    }
+    cen.use = 1;
+  }
   if (work_bytes < kpp_mlp_schedule_work_bytes(n)) return KPP_E_WORKSPACE;
   hipStream_t st = as_stream(stream);
   Work w = carve(work, n);
@@ -597,7 +617,7 @@ int kpp_mlp_schedule(const kpp_mlp* net, const void* X, int n, float mu_y, float
   uint8_t* key = (uint8_t*)p;
   int* counts = (int*)(p + ((size_t)n + 255) / 256 * 256);
   hipLaunchKernelGGL(k_decode, dim3((n + 255) / 256), dim3(256), 0, st, n, w.O, mu_y, sd_y, hmin,
-                     hmax, hstart, bucket, order ? key : nullptr);
+                     hmax, cen, hstart, bucket, order ? key : nullptr);
   if (order) {
     const int nblk = (n + SORT_CHUNK - 1) / SORT_CHUNK;
     hipLaunchKernelGGL(k_sort_count, dim3(nblk), dim3(SORT_THREADS), 0, st, n, key, nblk, counts);

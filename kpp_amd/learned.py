@@ -54,7 +54,7 @@ def _lib():
                                          vp]
         L.kpp_mlp_schedule_work_bytes.restype = cs
         L.kpp_mlp_schedule_work_bytes.argtypes = [ci]
-        L.kpp_mlp_schedule.argtypes = [vp, vp, ci, cf, cf, cd, cd, vp, vp, vp, vp, cs, vp]
+        L.kpp_mlp_schedule.argtypes = [vp, vp, ci, cf, cf, cd, cd, vp, vp, vp, vp, vp, cs, vp]
         L._mlp_bound = True
     return L
 
@@ -90,6 +90,10 @@ class StepModel:
     sd_y: float = 1.0
     h_scale: float = 1.0    # starting step = h_scale x the predicted first step
     edges: np.ndarray = field(default_factory=lambda: np.zeros(NBUCKET - 1))
+    # mean step count of each bucket in the training data: the launch order
+    # sorts by the expected count under the bucket softmax (None: by the
+    # expected bucket index)
+    centres: np.ndarray = None
     _work: object = None
 
     def __post_init__(self):
@@ -241,9 +245,12 @@ class StepModel:
         ws = self.work(int(_lib().kpp_mlp_schedule_work_bytes(n)))
         net = self.net()
         mu = float(self.mu_y + np.log10(self.h_scale))  # 10^(o*sd + mu) * h_scale
+        cen = None
+        if self.centres is not None:
+            cen = (ctypes.c_float * NBUCKET)(*[float(c) for c in self.centres])
         _check(_lib().kpp_mlp_schedule(ctypes.byref(net), _p(X), n, mu,
-                                       float(self.sd_y), float(hmin), float(hmax), _p(hs), _p(bk),
-                                       _p(order), _p(ws), ws.numel(), _stream()),
+                                       float(self.sd_y), float(hmin), float(hmax), cen, _p(hs),
+                                       _p(bk), _p(order), _p(ws), ws.numel(), _stream()),
                "kpp_mlp_schedule")
         return hs, bk, order
 
@@ -266,6 +273,7 @@ class StepModel:
         self.sd_y = float(lh.std()) or 1.0
         q = torch.quantile(nstp.double().cpu(), torch.linspace(0, 1, NBUCKET + 1, dtype=torch.float64)[1:-1])
         self.edges = q.numpy() + 0.5
+        self.centres = bucket_centres(nstp.cpu().numpy(), self.edges)
 
     # ---- persistence ----
     def save(self, path):
@@ -273,6 +281,8 @@ class StepModel:
                  mu=self.mu.cpu().numpy(), sd=self.sd.cpu().numpy(),
                  mu_y=np.float64(self.mu_y), sd_y=np.float64(self.sd_y), edges=self.edges,
                  h_scale=np.float64(self.h_scale),
+                 **({} if self.centres is None else {"centres": np.asarray(self.centres,
+                                                                          np.float32)}),
                  cfloor=np.float64(self.cfloor), in_pad=np.int32(self.in_pad))
 
     @classmethod
@@ -284,10 +294,25 @@ class StepModel:
                 mu=torch.tensor(z["mu"], dtype=torch.float32, device=device),
                 sd=torch.tensor(z["sd"], dtype=torch.float32, device=device),
                 mu_y=float(z["mu_y"]), sd_y=float(z["sd_y"]), edges=np.array(z["edges"]),
-                h_scale=float(z["h_scale"]) if "h_scale" in z.files else 1.0)
+                h_scale=float(z["h_scale"]) if "h_scale" in z.files else 1.0,
+                centres=np.array(z["centres"], np.float32) if "centres" in z.files else None)
         if m.in_pad != int(z["in_pad"]):
             raise ValueError("weights file does not match the mechanism's input width")
         return m
+
+
+def bucket_centres(nstp, edges):
+    """Mean step count of each bucket (the buckets of ``targets``), increasing.
+
+    An empty bucket takes the middle of its neighbours' edges."""
+    nstp = np.asarray(nstp, np.float64)
+    b = np.searchsorted(np.asarray(edges, np.float64), nstp, side="right")
+    e = np.concatenate([[nstp.min()], np.asarray(edges, np.float64), [nstp.max()]])
+    c = np.array([nstp[b == k].mean() if (b == k).any() else 0.5 * (e[k] + e[k + 1])
+                  for k in range(NBUCKET)])
+    for k in range(1, NBUCKET):  # strictly increasing, as the schedule requires
+        c[k] = max(c[k], np.nextafter(np.float32(c[k - 1]), np.float32(np.inf)))
+    return c.astype(np.float32)
 
 
 def log_steps(b, d, ws=None):
@@ -390,10 +415,27 @@ def main():
     ap.add_argument("--ncells", type=int, default=1048576)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--centres-for", default=None,
+                    help="instead of training: add bucket centres to this trained weights file "
+                         "from the step logs of the same seeds and training split, write --out")
     a = ap.parse_args()
     wl = workloads.WORKLOADS[a.workload]
     dev = torch.device("cuda:0")
     mech = wl(ncells=64, seed=0).mech
+    if a.centres_for:
+        model = StepModel.load(a.centres_for, dev)
+        ns = []
+        for s in [int(x) for x in a.seeds.split(",")]:
+            b = wl(ncells=a.ncells, seed=s)
+            ns.append(log_steps(b, b.to_device(dev))[1])
+        nstp = torch.cat(ns)
+        N = nstp.shape[0]
+        perm = torch.randperm(N, generator=torch.Generator().manual_seed(1)).to(dev)
+        tr = perm[int(N * 0.1):]  # the training split of train_model
+        model.centres = bucket_centres(nstp[tr].cpu().numpy(), model.edges)
+        model.save(a.out)
+        print(json.dumps({"centres": [float(c) for c in model.centres]}))
+        return
     model, metrics = train_model(mech, wl, [int(s) for s in a.seeds.split(",")], a.ncells, dev,
                                  epochs=a.epochs)
     model.save(a.out)

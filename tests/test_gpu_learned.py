@@ -116,12 +116,14 @@ def test_adam_step(gpu):
     assert np.array_equal(m.wb.cpu().float().numpy(), m.w.cpu().to(torch.bfloat16).float().numpy())
 
 
-def test_schedule_outputs(gpu):
+@pytest.mark.parametrize("centres", [None, [12.0, 14.0, 15.5, 17.5, 39.0, 46.5, 53.0, 63.5]])
+def test_schedule_outputs(gpu, centres):
     import torch
     m = _model(gpu, seed=8)
     n = 3000
     _, Xd = _inputs(m, n, 9)
     m.mu_y, m.sd_y = -1.0, 2.0
+    m.centres = None if centres is None else np.array(centres, np.float32)
     hs, bk, order = m.schedule(Xd, 1e-6, 3600.0)
     out = m.forward(Xd)
     torch.cuda.synchronize()
@@ -133,19 +135,29 @@ def test_schedule_outputs(gpu):
     assert np.array_equal(b, O[:, 1:9].argmax(1))
     o = order.cpu().numpy()
     assert np.array_equal(np.sort(o), np.arange(n))
-    # launch order: expected bucket under the softmax, most steps first, on 64
-    # levels (recomputed here in float64: a level can differ by one where the
-    # device's FP32 softmax rounds the other way)
+    # launch order: expected step count (bucket centres) or expected bucket
+    # index under the softmax, most steps first, on 64 levels between the
+    # first and last centre (recomputed here in float64: a level can differ by
+    # one where the device's FP32 softmax rounds the other way)
     lg = O[:, 1:9]
     p = np.exp(lg - lg.max(1, keepdims=True))
-    eb = (p * np.arange(8)).sum(1) / p.sum(1)
-    key = 63 - np.clip(np.floor(eb * 9.0 + 0.5), 0, 63).astype(int)
+    c = np.arange(8.0) if centres is None else np.array(centres, np.float64)
+    eb = (p * c).sum(1) / p.sum(1)
+    key = 63 - np.clip(np.floor((eb - c[0]) * 63.0 / (c[-1] - c[0]) + 0.5), 0, 63).astype(int)
     ko = key[o]
     assert np.all(np.diff(ko) >= -1)
     assert np.mean(np.diff(ko) >= 0) > 0.99
     # stable: within one key, cells keep their index order
     same = np.diff(ko) == 0
     assert np.mean(np.diff(o)[same] > 0) > 0.99
+
+
+def test_schedule_rejects_bad_centres(gpu):
+    m = _model(gpu, seed=8)
+    _, Xd = _inputs(m, 64, 9)
+    m.centres = np.array([1, 2, 3, 4, 4, 5, 6, 7], np.float32)  # not increasing
+    with pytest.raises(solver.KppError):
+        m.schedule(Xd, 1e-6, 3600.0)
 
 
 def test_features_match_oracle(gpu):

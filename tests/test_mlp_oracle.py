@@ -123,3 +123,20 @@ def test_adam_first_step_is_signed_lr():
     g = np.array([0.3, -4.0, 1e-3])
     w1, m1, m2 = R.adam(w, g, np.zeros(3), np.zeros(3), 1, 0.01)
     assert np.allclose(w1, w - 0.01 * np.sign(g), rtol=0, atol=1e-7)
+
+
+def test_bucket_centres():
+    from kpp_amd import learned
+    rng = np.random.default_rng(4)
+    nstp = np.concatenate([rng.integers(5, 20, 5000), rng.integers(30, 90, 3000)])
+    edges = np.quantile(nstp, np.linspace(0, 1, 9)[1:-1]) + 0.5
+    c = learned.bucket_centres(nstp, edges)
+    b = np.searchsorted(edges, nstp, side="right")
+    assert c.dtype == np.float32 and len(c) == 8
+    assert np.all(np.diff(c) > 0)
+    for k in range(8):
+        if (b == k).any():
+            assert abs(c[k] - nstp[b == k].mean()) < 1e-3 or k > 0 and c[k] > c[k - 1]
+    # an empty bucket sits between its edges
+    c2 = learned.bucket_centres(np.array([1, 2, 3, 50, 60]), np.array([1.5, 2.5, 3.5, 10.5, 20.5, 30.5, 55.5]))
+    assert np.all(np.diff(c2) > 0) and 10.5 <= c2[4] <= 20.5
