@@ -364,6 +364,32 @@ def main():
             accuracy["tuned_vs_plain"] = rel_errors(runs["tuned"]["C"][:, cells],
                                                     runs["plain"]["C"][:, cells], mech.nvar, at, rt)
 
+    # PCIe-inclusive rate: the batch's inputs (C, TEMP, CFACTOR, SUN, Tstart,
+    # Tend) copied in from pinned host memory and C copied back, around the
+    # top path's device time (not `value`, which starts with inputs in HBM)
+    ins = [C0, d["temp"], d["cfactor"], d["sun"], d["tstart"], d["tend"]]
+    hosts = [torch.empty(t.shape, dtype=t.dtype, pin_memory=True) for t in ins]
+    for h, t in zip(hosts, ins):
+        h.copy_(t)
+    scratch = [torch.empty_like(t) for t in ins]
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    torch.cuda.synchronize()
+    e0.record()
+    for t, h in zip(scratch, hosts):
+        t.copy_(h, non_blocking=True)
+    e1.record()
+    hosts[0].copy_(scratch[0], non_blocking=True)
+    e2.record()
+    torch.cuda.synchronize()
+    h2d, d2h = e0.elapsed_time(e1) / 1e3, e1.elapsed_time(e2) / 1e3
+    del scratch
+    pcie = {"value": n / (h2d + top["device_s"] + d2h), "unit": "cells/s",
+            "h2d_ms": h2d * 1e3, "d2h_ms": d2h * 1e3,
+            "bytes_in": int(sum(t.numel() * t.element_size() for t in ins)),
+            "bytes_out": int(C0.numel() * C0.element_size())}
+    log(f"[bench] PCIe-inclusive: H2D {h2d * 1e3:.1f} ms, D2H {d2h * 1e3:.1f} ms, "
+        f"{pcie['value']:.0f} cells/s")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the CPU oracle on a sample ...")
@@ -419,6 +445,7 @@ def main():
                    "speedup_vs_fp64_persistent": runs["persistent"]["wall"] / runs["mixed"]["wall"],
                    "all_ierr_1": runs["mixed"]["ok"]} if "mixed" in runs else None),
         "speedup_vs_plain": (value / (n / plain["wall"])) if plain else None,
+        "pcie_inclusive": pcie,
         "accuracy": accuracy,
         "cpu_baseline": cpu,
     }

@@ -1802,9 +1802,10 @@ void emit_hip(Model& md, const std::string& path) {
               "TV V, TF F, TR RCT, TJ JVS", "V, F, RCT, JVS", 0, st);
   }
   // KppDecomp unrolled: row-wise IKJ elimination of util/sutil.c:2-23 with
-  // the work row W kept in place (fill closure guarantees every updated
-  // column is in the row's pattern).  l = W[j]/U_jj, W[c] -= l*U_jc is
-  // bitwise the reference's a=-W[j]/U_jj; W[j]=-a; W[c]+=a*U_jc.
+  // the work row W held in locals named after the row's LU entries (fill
+  // closure guarantees every updated column is in the row's pattern).
+  // l = W[j]/U_jj, W[c] -= l*U_jc is bitwise the reference's a=-W[j]/U_jj;
+  // W[j]=-a; W[c]+=a*U_jc.
   // KppDecompChecked adds the singularity test of util/sutil.c:8 at the start
   // of each row and returns k+1 for the first row whose diagonal is zero.
   for (int checked = 0; checked < 2; checked++) {
@@ -1816,12 +1817,12 @@ void emit_hip(Model& md, const std::string& path) {
                         std::to_string(k + 1) + ";\n";
       std::map<int, int> pos;  // column -> position in row k
       for (int kk = S.lu_crow[k]; kk < S.lu_crow[k + 1]; kk++) pos[S.lu_icol[kk]] = kk;
-      if (!checked) {
+      {
         // Row k held in locals: every entry the elimination touches is loaded
         // first and stored once at the end, so no store of row k sits between
         // the loads of the pivot rows it reads (earlier rows only) and they
-        // can all be in flight together.  Same operations, in the same order
-        // per entry, as the statement form below.
+        // can all be in flight together.  Per entry the operations and their
+        // order are those of the IKJ loop: W[j] /= U_jj, then W[c] -= W[j]*U_jc.
         std::vector<int> touched;
         std::string ops;
         for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
@@ -1852,24 +1853,7 @@ void emit_hip(Model& md, const std::string& path) {
           t += "\n  }\n";
         }
         st.push_back({t, n_ops});
-        continue;
       }
-      for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
-        int j = S.lu_icol[kk];
-        t += "  { const auto l = G[" + std::to_string(kk) + "]/G[" + std::to_string(S.lu_diag[j]) +
-             "]; G[" + std::to_string(kk) + "] = l;";
-        n_ops++;
-        for (int jj = S.lu_diag[j] + 1; jj < S.lu_crow[j + 1]; jj++) {
-          int c = S.lu_icol[jj];
-          auto it = pos.find(c);
-          if (it == pos.end()) die("LU pattern not closed under elimination");
-          t += " G[" + std::to_string(it->second) + "] = G[" + std::to_string(it->second) +
-               "] - l*G[" + std::to_string(jj) + "];";
-          n_ops += 2;
-        }
-        t += " }\n";
-      }
-      st.push_back({t, n_ops});
     }
     routine("template <class TG>", checked ? "KppDecompChecked" : "KppDecomp", "TG G", "G",
             checked ? 1 : 0, st);

@@ -207,9 +207,13 @@ bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry
   if (lanes > 64) return false;
   if (lanes == 0) {
     // 64 cells per wavefront unless that leaves resident wavefront slots
-    // empty; then fewer (down to 16), so more wavefronts share the latency
+    // empty; then fewer (down to 16), so more wavefronts share the latency.
+    // Not for large mechanisms: with thousands of LU entries per cell, full
+    // wavefronts win (MCM, 16,384 cells on MI355X: 64 lanes 1.74 s, 32 lanes
+    // 1.78 s, 16 lanes 1.88 s, 8 lanes slower still).
+    const int min_lanes = (m->lu_nonzero > 4096) ? 64 : 16;
     lanes = 64;
-    while (lanes > 16 && (long)(ncells + lanes - 1) / lanes < resident) lanes /= 2;
+    while (lanes > min_lanes && (long)(ncells + lanes - 1) / lanes < resident) lanes /= 2;
   }
   g.lanes = lanes;
   if (variant == KPP_VARIANT_PERSISTENT) {
