@@ -37,6 +37,9 @@ ORACLE_BUILD = ORACLE / "build"
 MECHANISMS = {
     "small_strato": ("models/small_strato.def", []),
     "saprc99": ("models/saprc99.def", []),
+    # KPP's second SAPRC-99 model (models/saprcnov.*): same rate-law syntax, 88
+    # variable species, N2 among the fixed species
+    "saprcnov": ("models/saprcnov.def", []),
     # MCM isoprene export: rate code in Fortran 90 (examples/mcm/constants_mcm.f90,
     # inlined through #INLINE F90_RCONST), with the air density and its O2, N2,
     # H2O fractions and the solar zenith angle from examples/mcm/driver_mcm.f90
@@ -66,7 +69,7 @@ HIP_FLAGS = [
 # (256 VGPRs) and 2.07 s at 1; MCM on 16,384 cells 2.35 s at 4 (its 16,384
 # cells fill one wavefront per SIMD at most), 2.03 s at 2 and 1.88 s at 1
 # (VGPRs plus AGPRs for spills).  small_strato fits in registers at 4.
-WAVES_PER_EU = {"small_strato": 4, "saprc99": 2, "mcm": 1}
+WAVES_PER_EU = {"small_strato": 4, "saprc99": 2, "saprcnov": 2, "mcm": 1}
 
 
 def kpp_home() -> Path:

@@ -113,9 +113,9 @@ def small_strato_tod(ncells=4096, seed=1, cells=None, rtol=1e-3, atol=1.0, metho
                  {"workload": "small_strato_tod"})
 
 
-def _saprc99(ncells, seed, cells, uneven, rtol, atol, method):
+def _saprc99(ncells, seed, cells, uneven, rtol, atol, method, mech="saprc99"):
     cells = np.arange(ncells, dtype=np.int64) if cells is None else np.asarray(cells)
-    vals, _, st = mech_init("saprc99")
+    vals, _, st = mech_init(mech)
     nvar = st["NVAR"]
     T = 260.0 + 50.0 * uniform(seed, F_TEMP, cells)
     p_hpa = 500.0 + 513.0 * uniform(seed, F_PRES, cells)
@@ -128,9 +128,9 @@ def _saprc99(ncells, seed, cells, uneven, rtol, atol, method):
     else:
         sun = np.ones(len(cells))
     tstart = np.full(len(cells), 12.0 * 3600.0)
-    return Batch("saprc99", method, C, T, cf, sun, solver.SUN_FIXED, tstart, tstart + 3600.0,
+    return Batch(mech, method, C, T, cf, sun, solver.SUN_FIXED, tstart, tstart + 3600.0,
                  np.array([atol]), np.array([rtol]), solver.default_ipar(method), np.zeros(20),
-                 cells, {"workload": "saprc99_uneven" if uneven else "saprc99_day"})
+                 cells, {"workload": f"{mech}_uneven" if uneven else f"{mech}_day"})
 
 
 def saprc99_day(ncells=65536, seed=2, cells=None, rtol=1e-3, atol=1.0, method=solver.RODAS3):
@@ -140,6 +140,14 @@ def saprc99_day(ncells=65536, seed=2, cells=None, rtol=1e-3, atol=1.0, method=so
 def saprc99_uneven(ncells=1048576, seed=3, cells=None, rtol=1e-3, atol=1.0,
                    method=solver.RODAS3):
     return _saprc99(ncells, seed, cells, True, rtol, atol, method)
+
+
+def saprcnov_uneven(ncells=65536, seed=9, cells=None, rtol=1e-3, atol=1.0,
+                    method=solver.RODAS3):
+    """models/saprcnov (KPP's other SAPRC-99 variant: 88 variable species, N2 fixed):
+    the saprc99_uneven recipe -- .def ppm values x 10^U(-1,1) x CFACTOR per cell,
+    T ~ U(260, 310) K, p ~ U(500, 1013) hPa, half the cells at night."""
+    return _saprc99(ncells, seed, cells, True, rtol, atol, method, mech="saprcnov")
 
 
 # MCM isoprene example (examples/mcm/driver_mcm.f90): air density M = 2.5E19
@@ -173,5 +181,6 @@ WORKLOADS = {
     "small_strato_tod": small_strato_tod,
     "saprc99_day": saprc99_day,
     "saprc99_uneven": saprc99_uneven,
+    "saprcnov_uneven": saprcnov_uneven,
     "mcm_mixed": mcm_mixed,
 }

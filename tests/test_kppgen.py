@@ -146,13 +146,17 @@ def _parse_eqn(path):
     return eqs
 
 
-@pytest.fixture(scope="module")
-def saprc99_dense(kpp_home):
-    eqs = _parse_eqn(kpp_home / "models" / "saprc99.eqn")
-    L = ref.lib("saprc99")
-    s = ref.sizes("saprc99")
+def _dense(kpp_home, mech):
+    eqs = _parse_eqn(kpp_home / "models" / f"{mech}.eqn")
+    L = ref.lib(mech)
+    s = ref.sizes(mech)
     names = [L.ros_ref_species_name(k).decode().upper() for k in range(s["NSPEC"])]
     return eqs, names, s
+
+
+@pytest.fixture(scope="module")
+def saprc99_dense(kpp_home):
+    return _dense(kpp_home, "saprc99")
 
 
 def _dense_fun_jac(eqs, names, s, C, R):
@@ -193,9 +197,17 @@ def test_saprc99_sizes(saprc99_dense):
     assert set(names[s["NVAR"]:]) == {"AIR", "O2", "H2O", "H2", "CH4"}
 
 
-def test_saprc99_fun_and_jac_match_dense_model(saprc99_dense):
-    eqs, names, s = saprc99_dense
-    L = ref.lib("saprc99")
+def test_saprcnov_sizes(kpp_home):
+    eqs, names, s = _dense(kpp_home, "saprcnov")
+    # models/saprcnov.spc declares N2 fixed, but no equation uses it
+    assert s["NVAR"] == 88 and s["NFIX"] == 5 and s["NREACT"] == 235 == len(eqs)
+    assert set(names[s["NVAR"]:]) == {"AIR", "O2", "H2O", "H2", "CH4"}
+
+
+@pytest.mark.parametrize("mech", ["saprc99", "saprcnov"])
+def test_saprc_fun_and_jac_match_dense_model(kpp_home, mech):
+    eqs, names, s = _dense(kpp_home, mech)
+    L = ref.lib(mech)
     nv, nf, nr = s["NVAR"], s["NFIX"], s["NREACT"]
     icol = np.zeros(s["LU_NONZERO"], np.int32)
     crow = np.zeros(nv + 1, np.int32)
@@ -220,13 +232,14 @@ def test_saprc99_fun_and_jac_match_dense_model(saprc99_dense):
         assert np.allclose(Jg, J, rtol=1e-12, atol=1e-12 * np.abs(J).max())
 
 
-def test_saprc99_pattern_invariants():
-    s = ref.sizes("saprc99")
+@pytest.mark.parametrize("mech", ["saprc99", "saprcnov"])
+def test_saprc_pattern_invariants(mech):
+    s = ref.sizes(mech)
     nv = s["NVAR"]
     icol = np.zeros(s["LU_NONZERO"], np.int32)
     crow = np.zeros(nv + 1, np.int32)
     diag = np.zeros(nv + 1, np.int32)
-    ref.lib("saprc99").ros_ref_lu_pattern(_p(icol), _p(crow), _p(diag))
+    ref.lib(mech).ros_ref_lu_pattern(_p(icol), _p(crow), _p(diag))
     P = np.zeros((nv, nv), bool)
     for i in range(nv):
         cols = icol[crow[i]:crow[i + 1]]
