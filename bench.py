@@ -114,11 +114,18 @@ def copy_probe(nbytes=1 << 30, reps=3):
     import torch
     a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda:0").fill_(1.0)
     b = torch.empty_like(a)
+    evs = []
     for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
         b.copy_(a)
+        e1.record()
+        evs.append((e0, e1))
     torch.cuda.synchronize()
+    ms = min(e0.elapsed_time(e1) for e0, e1 in evs[1:] or evs)
+    # the HBM rate a plain device copy reaches (read + write bytes per second)
     print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
-                      "reps": reps}))
+                      "reps": reps, "best_ms": ms, "hbm_gbs": 2 * nbytes / ms / 1e6}))
 
 
 def measured_traffic(workload, ncells):
