@@ -331,10 +331,16 @@ def main():
     if args.lu_fp32:
         runs["mixed"] = timed(solver.PERSISTENT, False, max(1, args.steps), args.warmup,
                               "persistent, FP32 LU + FP64 refinement", lu_fp32=1)
+    # small mechanisms: the kernel that keeps each cell's working set in VGPRs
+    reg_ok = solver.workspace_bytes(b.mech, b.method, n, variant=solver.REGISTER) > 0
+    if reg_ok and model is None and not args.lu_fp32:
+        runs["register"] = timed(solver.REGISTER, False, max(1, args.steps), args.warmup,
+                                 "register kernel (working set in VGPRs)")
     if not args.no_plain:
         runs["plain"] = timed(solver.PLAIN, False, max(1, args.steps), max(0, args.warmup), "plain")
-    top = runs["tuned"] if model is not None else (runs["mixed"] if args.lu_fp32
-                                                    else runs["persistent"])
+    top = runs["tuned"] if model is not None else (
+        runs["mixed"] if args.lu_fp32 else runs.get("register", runs["persistent"]))
+    top_register = model is None and not args.lu_fp32 and "register" in runs
 
     # roofline of the persistent kernel on the tuned run's attempts
     top_mixed = (model is None and args.lu_fp32)
@@ -404,6 +410,16 @@ def main():
         log(f"[bench] cpu baseline {cpu['value']:.0f} cells/s on {cpu['cores']} threads")
     value = n / top["wall"]
     achieved = nbytes / kern_t / 1e9
+    if top_register:
+        # no tile traffic: the cell's working set stays in VGPRs, so the bound
+        # is FP64 vector arithmetic (the emitted operation counts per attempt)
+        roof = {"bound": "fp64", "achieved": nflops / kern_t / 1e12, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": nflops / kern_t / 1e12 / FP64_PEAK_TFLOPS,
+                "traffic": None, "kernel": f"ros_reg_kernel<kppm_{b.mech},"
+                                           f"{solver.METHOD_STAGES[b.method]}>",
+                "kernel_ms": kern_t * 1e3, "algorithmic_flops": nflops}
+    else:
+        roof = None
     plain = runs.get("plain")
     line = {
         "metric": "cells/s integrated 1 h (saprc99, RTOL 1e-3): tuned vs plain GPU vs CPU",
@@ -426,9 +442,11 @@ def main():
                             "work-stealing kernel, recorded as one graph"
                             if model is not None else
                             ("persistent work-stealing kernel, FP32 LU + FP64 refinement"
-                             if args.lu_fp32 else "persistent work-stealing kernel")),
+                             if args.lu_fp32 else
+                             ("register kernel: one lane per cell, working set in VGPRs"
+                              if top_register else "persistent work-stealing kernel"))),
                    "all_ierr_1": top["ok"], "attempts_per_cell": attempts / n},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": roof or {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, n),
                      "kernel": f"ros_persistent_kernel<kppm_{b.mech},"
@@ -442,6 +460,10 @@ def main():
         "plain": ({"value": n / plain["wall"], "kernel_ms": plain["kernel_s"] * 1e3,
                    "attempts_per_cell": float(plain["istat"][2].mean()),
                    "all_ierr_1": plain["ok"]} if plain else None),
+        "register": ({"value": n / runs["register"]["wall"],
+                      "kernel_ms": runs["register"]["kernel_s"] * 1e3,
+                      "attempts_per_cell": float(runs["register"]["istat"][2].mean()),
+                      "all_ierr_1": runs["register"]["ok"]} if "register" in runs else None),
         "persistent_default_start": {"value": n / runs["persistent"]["wall"],
                                      "attempts_per_cell": float(runs["persistent"]["istat"][2].mean())},
         "mixed": ({"value": n / runs["mixed"]["wall"],

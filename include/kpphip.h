@@ -44,6 +44,9 @@ extern "C" {
 
 #define KPP_VARIANT_PLAIN 0      /* one lane per cell, cells in launch order */
 #define KPP_VARIANT_PERSISTENT 1 /* persistent work-stealing wavefronts */
+#define KPP_VARIANT_REGISTER 2   /* one lane per cell, working set in registers: small
+                                    mechanisms only (<= 64 LU entries and reactions),
+                                    else the call returns KPP_E_ARG */
 
 /* ---- mechanisms compiled into the library ---- */
 int kpp_mech_count(void);
@@ -90,7 +93,7 @@ typedef struct {
 } kpp_batch;
 
 typedef struct {
-  int variant;    /* KPP_VARIANT_PLAIN / KPP_VARIANT_PERSISTENT */
+  int variant;    /* KPP_VARIANT_PLAIN / _PERSISTENT / _REGISTER */
   int nblocks;    /* persistent: wavefronts in the grid (0 = library default) */
   int refill_min; /* persistent: refill when this many lanes are idle (0 = default 1) */
   int lanes;      /* cells per wavefront, 1..64 (0 = library default: 64, fewer for

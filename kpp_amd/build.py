@@ -68,8 +68,9 @@ HIP_FLAGS = [
 # 1.85 s at 4 waves/SIMD (128 VGPRs, 2.6 KB of spills per lane), 1.71 s at 2
 # (256 VGPRs) and 2.07 s at 1; MCM on 16,384 cells 2.35 s at 4 (its 16,384
 # cells fill one wavefront per SIMD at most), 2.03 s at 2 and 1.88 s at 1
-# (VGPRs plus AGPRs for spills).  small_strato fits in registers at 4.
-WAVES_PER_EU = {"small_strato": 4, "saprc99": 2, "saprcnov": 2, "mcm": 1}
+# (VGPRs plus AGPRs for spills).  small_strato: 2, so that its register
+# kernel (the whole working set in VGPRs, 71-86 doubles per lane) fits.
+WAVES_PER_EU = {"small_strato": 2, "saprc99": 2, "saprcnov": 2, "mcm": 1}
 
 
 def kpp_home() -> Path:

@@ -196,7 +196,10 @@ bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry
   const int rows = mixed ? mx->rows(S) : m->rows(S);
   if (rows <= 0) return false;
   const int variant = l ? l->variant : KPP_VARIANT_PLAIN;
-  if (variant != KPP_VARIANT_PLAIN && variant != KPP_VARIANT_PERSISTENT) return false;
+  if (variant != KPP_VARIANT_PLAIN && variant != KPP_VARIANT_PERSISTENT &&
+      variant != KPP_VARIANT_REGISTER)
+    return false;
+  if (variant == KPP_VARIANT_REGISTER && (mixed || !m->fits_registers)) return false;
   if (l && l->lu_fp32 != 0 && l->lu_fp32 != 1) return false;
   int cus = device_cus();
   if (cus <= 0) cus = 256;
@@ -227,7 +230,8 @@ bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry
     g.refill_min = 0;
   }
   g.tol_bytes = align256(2 * (size_t)m->nvar * sizeof(double));
-  g.tile_bytes = (size_t)rows * 64 * sizeof(double);
+  // the register kernel keeps the working set in VGPRs: no tiles
+  g.tile_bytes = (variant == KPP_VARIANT_REGISTER) ? 0 : (size_t)rows * 64 * sizeof(double);
   // persistent kernel: rate constants of every cell from a pre-pass
   g.rpre_bytes = (variant == KPP_VARIANT_PERSISTENT)
                      ? align256((size_t)m->nreact * (size_t)ncells * sizeof(double)) : 0;

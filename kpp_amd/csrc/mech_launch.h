@@ -76,6 +76,11 @@ hipError_t launch_s(const RosParams& p, int variant, int nblocks, hipStream_t st
     if (p.rpre)
       hipLaunchKernelGGL(ros_rates_prepass<M>, dim3((p.ncells + 255) / 256), dim3(256), 0, st, p);
     hipLaunchKernelGGL((ros_persistent_kernel<M, S, MIX>), dim3(nblocks), dim3(TILE), 0, st, p);
+  } else if (variant == KPP_VARIANT_REGISTER) {
+    if constexpr (fits_registers<M>() && !MIX)
+      hipLaunchKernelGGL((ros_reg_kernel<M, S>), dim3(nblocks), dim3(TILE), 0, st, p);
+    else
+      return hipErrorInvalidValue;
   } else {
     hipLaunchKernelGGL((ros_plain_kernel<M, S, MIX>), dim3(nblocks), dim3(TILE), 0, st, p);
   }
@@ -97,11 +102,17 @@ hipError_t launch_mech(const RosParams& p, int variant, int nblocks, hipStream_t
 template <class M, int S, bool MIX>
 int occupancy_s(int variant) {
   int n = 0;
-  hipError_t e =
-      (variant == KPP_VARIANT_PERSISTENT)
-          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_persistent_kernel<M, S, MIX>,
-                                                         TILE, 0)
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_plain_kernel<M, S, MIX>, TILE, 0);
+  hipError_t e;
+  if (variant == KPP_VARIANT_PERSISTENT) {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_persistent_kernel<M, S, MIX>, TILE, 0);
+  } else if (variant == KPP_VARIANT_REGISTER) {
+    if constexpr (fits_registers<M>() && !MIX)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_reg_kernel<M, S>, TILE, 0);
+    else
+      return 0;
+  } else {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_plain_kernel<M, S, MIX>, TILE, 0);
+  }
   return e == hipSuccess ? n : 0;
 }
 template <class M, bool MIX>
@@ -159,6 +170,7 @@ MechOps make_ops(const char* name) {
   o.n_sun_rates = M::N_SUN_RATES;
   o.rates_use_state = M::RATES_USE_STATE ? 1 : 0;
   o.n_state_rates = M::N_STATE_RATES;
+  o.fits_registers = fits_registers<M>() ? 1 : 0;
   o.species = D::names();
   o.lu_icol = D::icol();
   o.lu_crow = D::crow();

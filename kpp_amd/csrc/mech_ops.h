@@ -7,6 +7,7 @@
 
 #define KPP_VARIANT_PLAIN 0
 #define KPP_VARIANT_PERSISTENT 1
+#define KPP_VARIANT_REGISTER 2
 
 #define KPP_COMP_FUN 0
 #define KPP_COMP_JAC 1
@@ -20,6 +21,7 @@ struct MechOps {
   const char* name;
   int nvar, nfix, nspec, nreact, nonzero, lu_nonzero;
   int rates_use_time, n_sun_rates, rates_use_state, n_state_rates;
+  int fits_registers;  // KPP_VARIANT_REGISTER is compiled for this mechanism
   const char* const* species;
   const int *lu_icol, *lu_crow, *lu_diag;
   const double* init;  // Initialize(): #INITVALUES times CFACTOR, [NSPEC]

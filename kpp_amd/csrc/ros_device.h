@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "kpphip_params.h"
 
 // Build knobs (set per mechanism by kpp_amd/build.py):
@@ -199,6 +201,100 @@ KPPHIP_MODEL_CALL void call_rates_fixed(double TEMP, double CFACTOR, dptr R) {
   M::rates_fixed(TEMP, CFACTOR, Acc{R});
 }
 
+// Storage of a lane's working set.
+//
+// TileMem: one column of a 64-lane tile in global memory (every mechanism);
+// the model routines are called out of line on row pointers.
+template <class M>
+struct TileMem {
+  using A = Acc;
+  static constexpr bool kRegisters = false;
+  double* base;  // lane base pointer inside its tile
+  __device__ __forceinline__ A at(int row) const { return Acc{(gdouble*)(base + row * TILE)}; }
+  __device__ __forceinline__ static void fun(A V, A F, A R, A out) {
+    call_fun<M>(V.b, F.b, R.b, out.b);
+  }
+  __device__ __forceinline__ static int jac_ghimj(A V, A F, A R, double ghinv, A G) {
+    return call_jac_ghimj<M>(V.b, F.b, R.b, ghinv, G.b);
+  }
+  __device__ __forceinline__ static void decomp(A G) { call_decomp<M>(G.b); }
+  __device__ __forceinline__ static void solve(A G, A X) { call_solve<M>(G.b, X.b); }
+  __device__ __forceinline__ static void rates_sun(double T, double CF, double SUN, double TIME,
+                                                   A R) {
+    call_rates_sun<M>(T, CF, SUN, TIME, R.b);
+  }
+  __device__ __forceinline__ static void rates_state(double T, double CF, double SUN,
+                                                     double TIME, A V, A F, A R) {
+    call_rates_state<M>(T, CF, SUN, TIME, V.b, F.b, R.b);
+  }
+  __device__ __forceinline__ static void rates_fixed(double T, double CF, A R) {
+    call_rates_fixed<M>(T, CF, R.b);
+  }
+};
+
+// RegMem: a private array of the lane (small mechanisms).  Every element
+// offset is a compile-time constant -- the model routines are inlined and the
+// integrator's loops over species and reactions unrolled -- so the array
+// lives in VGPRs and an attempt touches no memory but the cell's inputs and
+// outputs.  Same operations in the same order as with TileMem.
+struct RegAcc {
+  using value_type = double;
+  double* b;
+  __device__ __forceinline__ double& operator[](int i) const { return b[i]; }
+};
+template <class M, int ROWS>
+struct RegMem {
+  using A = RegAcc;
+  static constexpr bool kRegisters = true;
+  mutable double a[ROWS];
+  __device__ __forceinline__ A at(int row) const { return RegAcc{a + row}; }
+  __device__ __forceinline__ static void fun(A V, A F, A R, A out) { M::Fun(V, F, R, out); }
+  __device__ __forceinline__ static int jac_ghimj(A V, A F, A R, double ghinv, A G) {
+    return M::JacGhimj(V, F, R, ghinv, G);
+  }
+  __device__ __forceinline__ static void decomp(A G) { M::KppDecomp(G); }
+  __device__ __forceinline__ static void solve(A G, A X) { M::KppSolve(G, X); }
+  __device__ __forceinline__ static void rates_sun(double T, double CF, double SUN, double TIME,
+                                                   A R) {
+    M::rates_sun(T, CF, SUN, TIME, R);
+  }
+  __device__ __forceinline__ static void rates_state(double T, double CF, double SUN,
+                                                     double TIME, A V, A F, A R) {
+    M::rates_state(T, CF, SUN, TIME, V, F, R);
+  }
+  __device__ __forceinline__ static void rates_fixed(double T, double CF, A R) {
+    M::rates_fixed(T, CF, R);
+  }
+};
+
+// Mechanisms whose working set is small enough for RegMem (at most this many
+// LU entries: small_strato has 19; saprc99 920).
+template <class M>
+constexpr bool fits_registers() {
+  return M::LU_NONZERO <= 64 && M::NREACT <= 64;
+}
+
+// for (i = 0; i < N; i++) f(i): fully unrolled for RegMem (compile-time
+// indices), an ordinary loop otherwise; the calls run in increasing i either way.
+template <class F, int... I>
+__device__ __forceinline__ void vloop_seq(F& f, std::integer_sequence<int, I...>) {
+  (f(I), ...);
+}
+template <bool Unroll, int N, class F>
+__device__ __forceinline__ void vloop(F&& f) {
+  if constexpr (Unroll) {
+    vloop_seq(f, std::make_integer_sequence<int, N>{});
+  } else {
+    for (int i = 0; i < N; i++) f(i);
+  }
+}
+
+// stage(1), ..., stage(S) as S separate calls with constant arguments
+template <class F, int... I>
+__device__ __forceinline__ void stages(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I + 1>{}), ...);
+}
+
 // Per-lane integration state between attempts.
 struct CellState {
   int cell;
@@ -210,14 +306,17 @@ struct CellState {
   int Nfb;  // attempts that fell back from the FP32 to the FP64 factorisation
 };
 
-template <class M, int S, bool MIX = false>
+template <class M, int S, bool MIX = false, class Mem = TileMem<M>>
 struct Cell {
   using L = Layout<M, S, MIX>;
-  double* base;  // lane base pointer inside its tile
-  __device__ __forceinline__ Acc at(int row) const { return Acc{(gdouble*)(base + row * TILE)}; }
-  // FP32 rows: element i of this lane at float index 2*row*TILE + i*TILE
+  using A = typename Mem::A;
+  static constexpr bool U = Mem::kRegisters;  // unroll the species/reaction loops
+  static_assert(!(MIX && U), "the mixed-precision integrator works on tiles");
+  Mem mem;
+  __device__ __forceinline__ A at(int row) const { return mem.at(row); }
+  // FP32 rows (tiles only): element i of this lane at float index 2*row*TILE + i*TILE
   __device__ __forceinline__ AccF at32(int row) const {
-    return AccF{(gfloat*)((float*)(base - lane_of()) + 2 * row * TILE + lane_of())};
+    return AccF{(gfloat*)((float*)(mem.base - lane_of()) + 2 * row * TILE + lane_of())};
   }
   __device__ __forceinline__ int lane_of() const { return (int)(threadIdx.x & (TILE - 1)); }
 
@@ -233,9 +332,9 @@ struct Cell {
     if (p.rates_given) return;
     const bool tod = p.sun_mode == KPP_SUN_TOD;
     const double sun = tod ? update_sun(T) : s.SUN;
-    if (tod || M::RATES_USE_TIME) call_rates_sun<M>(s.TEMP, s.CFACTOR, sun, T, at(L::RCT).b);
+    if (tod || M::RATES_USE_TIME) Mem::rates_sun(s.TEMP, s.CFACTOR, sun, T, at(L::RCT));
     if (M::RATES_USE_STATE && (accepted || tod || M::RATES_USE_TIME))
-      call_rates_state<M>(s.TEMP, s.CFACTOR, sun, T, at(L::Y).b, at(L::FIX).b, at(L::RCT).b);
+      Mem::rates_state(s.TEMP, s.CFACTOR, sun, T, at(L::Y), at(L::FIX), at(L::RCT));
   }
 
   // Load a cell into this lane's slot; the set-up part of Rosenbrock()
@@ -246,19 +345,20 @@ struct Cell {
     s.TEMP = p.temp[cell];
     s.CFACTOR = p.cfactor[cell];
     s.SUN = p.sun ? p.sun[cell] : 0.0;
-    Acc Y = at(L::Y), F = at(L::FIX), R = at(L::RCT);
-    for (int i = 0; i < M::NVAR; i++) Y[i] = p.C[i * ld + cell];
-    for (int i = 0; i < M::NFIX; i++) F[i] = p.C[(M::NVAR + i) * ld + cell];
+    A Y = at(L::Y), F = at(L::FIX), R = at(L::RCT);
+    vloop<U, M::NVAR>([&](int i) { Y[i] = p.C[i * ld + cell]; });
+    vloop<U, M::NFIX>([&](int i) { F[i] = p.C[(M::NVAR + i) * ld + cell]; });
     if (p.rates_given) {
-      for (int i = 0; i < M::NREACT; i++) R[i] = p.rconst[i * ld + cell];
+      vloop<U, M::NREACT>([&](int i) { R[i] = p.rconst[i * ld + cell]; });
     } else if (p.rpre) {
       // computed by ros_rates_prepass with the same inputs and formulae
       const size_t n = (size_t)p.ncells;
-      for (int i = 0; i < M::NREACT; i++) R[i] = p.rpre[i * n + cell];
+      vloop<U, M::NREACT>([&](int i) { R[i] = p.rpre[i * n + cell]; });
     } else {
-      call_rates_fixed<M>(s.TEMP, s.CFACTOR, R.b);
+      if constexpr (U) vloop<U, M::NREACT>([&](int i) { R[i] = 0.0; });  // NUMBER-rate slots
+      Mem::rates_fixed(s.TEMP, s.CFACTOR, R);
       if (p.sun_mode == KPP_SUN_FIXED && !M::RATES_USE_TIME)
-        call_rates_sun<M>(s.TEMP, s.CFACTOR, s.SUN, 0.0, R.b);
+        Mem::rates_sun(s.TEMP, s.CFACTOR, s.SUN, 0.0, R);
     }
     s.Nfun = p.stat0[0]; s.Njac = p.stat0[1]; s.Nstp = p.stat0[2]; s.Nacc = p.stat0[3];
     s.Nrej = p.stat0[4]; s.Ndec = p.stat0[5]; s.Nsol = p.stat0[6]; s.Nsng = p.stat0[7];
@@ -282,8 +382,8 @@ struct Cell {
   __device__ void finish(const RosParams& p, const CellState& s, int ierr, double Texit) const {
     const size_t ld = (size_t)p.ldc;
     const int cell = s.cell;
-    Acc Y = at(L::Y);
-    for (int i = 0; i < M::NVAR; i++) p.C[i * ld + cell] = Y[i];
+    A Y = at(L::Y);
+    vloop<U, M::NVAR>([&](int i) { p.C[i * ld + cell] = Y[i]; });
     p.ierr[cell] = ierr;
     if (p.istat) {
       int* q = p.istat;
@@ -304,8 +404,8 @@ struct Cell {
     constexpr int NVAR = M::NVAR;
     const RosCoef& m = p.coef;
     const double Roundoff = p.Roundoff;
-    Acc Y = at(L::Y), F = at(L::FIX), R = at(L::RCT), Fcn0 = at(L::FCN0), Fcn = at(L::FCN),
-        dFdT = at(L::DFDT), Ynew = at(L::YNEW), K0 = at(L::K), G = at(L::G);
+    A Y = at(L::Y), F = at(L::FIX), R = at(L::RCT), Fcn0 = at(L::FCN0), Fcn = at(L::FCN),
+      dFdT = at(L::DFDT), Ynew = at(L::YNEW), K0 = at(L::K), G = at(L::G);
     const bool skip_dfdt = p.dfdt_zero;
     const double Dir = (double)s.Direction;
 
@@ -320,7 +420,7 @@ struct Cell {
       s.H = dmin(s.H, abs_diff_macro(s.Tend, s.T));
       // Fcn0 = F(T, Y)
       rates_at(p, s, s.T, true);
-      call_fun<M>(Y.b, F.b, R.b, Fcn0.b);
+      Mem::fun(Y, F, R, Fcn0);
       s.Nfun++;
       // dF/dT by forward difference (int/rosenbrock.c:663-682)
       if (!p.autonomous) {
@@ -328,12 +428,12 @@ struct Cell {
         if (!skip_dfdt) {
           const double Delta = sqrt(Roundoff) * dmax(KPP_DELTAMIN, dabs(s.T));
           rates_at(p, s, s.T + Delta, false);
-          call_fun<M>(Y.b, F.b, R.b, dFdT.b);
+          Mem::fun(Y, F, R, dFdT);
           const double inv = 1.0 / Delta;
-          for (int i = 0; i < NVAR; i++) {
+          vloop<U, NVAR>([&](int i) {
             const double d = dFdT[i] + (-1.0) * Fcn0[i];
             dFdT[i] = (inv == 1.0) ? d : inv * d;
-          }
+          });
         }
       }
       // Jac0 at (T, Y): counted here, evaluated inside every Ghimj build
@@ -348,7 +448,7 @@ struct Cell {
     int Nconsecutive = 0;
     for (;;) {
       const double ghinv = 1.0 / (Dir * s.H * m.Gamma[0]);
-      const int ising = call_jac_ghimj<M>(Y.b, F.b, R.b, ghinv, G.b);
+      const int ising = Mem::jac_ghimj(Y, F, R, ghinv, G);
       s.Ndec++;
       if (ising == 0) break;
       s.Nsng++;
@@ -368,13 +468,13 @@ struct Cell {
       for (int i = 0; i < M::LU_NONZERO; i++) G32[i] = (float)G[i];
       call_decomp32<M>(G32.b);
     } else {
-      call_decomp<M>(G.b);
+      Mem::decomp(G);
     }
 
     const double H = s.H;
-#pragma unroll
-    for (int istage = 1; istage <= S; istage++) {
-      Acc Ki{K0.b + (istage - 1) * NVAR * TILE};
+    auto stage = [&](auto stage_no) __attribute__((always_inline)) {
+      constexpr int istage = decltype(stage_no)::value;
+      A Ki = at(L::K + (istage - 1) * NVAR);
       // Fcn of this stage: output of the latest stage q<=istage with NewF,
       // or Fcn0 (stage 1 copies Fcn0; int/rosenbrock.c:547-560)
       bool from_fcn0 = true;
@@ -382,7 +482,7 @@ struct Cell {
       for (int q = 2; q <= istage; q++)
         if (m.NewF[q - 1]) from_fcn0 = false;
       if (istage > 1 && m.NewF[istage - 1]) {
-        for (int i = 0; i < NVAR; i++) {
+        vloop<U, NVAR>([&](int i) {
           double y = Y[i];
 #pragma unroll
           for (int j = 1; j <= istage - 1; j++) {
@@ -390,13 +490,14 @@ struct Cell {
             if (a != 0.0) y = y + a * K0[(j - 1) * NVAR + i];
           }
           Ynew[i] = y;
-        }
+        });
         const double Tau = s.T + m.Alpha[istage - 1] * Dir * H;
         rates_at(p, s, Tau, false);
-        call_fun<M>(Ynew.b, F.b, R.b, Fcn.b);
+        Mem::fun(Ynew, F, R, Fcn);
         s.Nfun++;
       }
-      Acc Fc = from_fcn0 ? Fcn0 : Fcn;
+      // (registers: select the value, not the row, so every index stays static)
+      A Fc = from_fcn0 ? Fcn0 : Fcn;
       double HC[S > 1 ? S - 1 : 1];
 #pragma unroll
       for (int j = 1; j <= istage - 1; j++)
@@ -404,14 +505,24 @@ struct Cell {
       const double HG = Dir * H * m.Gamma[istage - 1];
       const bool use_g = (!p.autonomous) && (m.Gamma[istage - 1] != 0.0) && !skip_dfdt &&
                          (HG != 0.0);
-      for (int i = 0; i < NVAR; i++) {
-        double k = Fc[i];
+      vloop<U, NVAR>([&](int i) {
+        double k;
+        if constexpr (U) {
+          // both rows read, one chosen bit for bit (a select of the values,
+          // which the compiler cannot turn back into a select of rows)
+          const unsigned long long a0 = __double_as_longlong(Fcn0[i]);
+          const unsigned long long a1 = __double_as_longlong(Fcn[i]);
+          const unsigned long long sel = from_fcn0 ? ~0ull : 0ull;
+          k = __longlong_as_double((long long)((a0 & sel) | (a1 & ~sel)));
+        } else {
+          k = Fc[i];
+        }
 #pragma unroll
         for (int j = 1; j <= istage - 1; j++)
           if (HC[j - 1] != 0.0) k = k + HC[j - 1] * K0[(j - 1) * NVAR + i];
         if (use_g) k = k + HG * dFdT[i];
         Ki[i] = k;
-      }
+      });
       if constexpr (MIX) {
        if (!lu64) {
         // x1 = FP32 solve of b; r = b - A x1 in FP64; x = x1 + FP32 solve of r
@@ -437,20 +548,22 @@ struct Cell {
           // this and the remaining stages use the FP64 factorisation
           lu64 = true;
           s.Nfb++;
-          call_decomp<M>(G.b);
-          call_solve<M>(G.b, Ki.b);
+          Mem::decomp(G);
+          Mem::solve(G, Ki);
         }
        } else {
-        call_solve<M>(G.b, Ki.b);
+        Mem::solve(G, Ki);
        }
       } else {
-        call_solve<M>(G.b, Ki.b);
+        Mem::solve(G, Ki);
       }
       s.Nsol++;
-    }
+    };
+    // one copy of the stage per istage (a compile-time constant in each)
+    stages(stage, std::make_integer_sequence<int, S>{});
     // new solution, error estimate and ros_ErrorNorm (:582-591, :632-659)
     double Err = 0.0;
-    for (int i = 0; i < NVAR; i++) {
+    vloop<U, NVAR>([&](int i) {
       const double y = Y[i];
       double yn = y, ye = 0.0;
 #pragma unroll
@@ -464,7 +577,7 @@ struct Cell {
       const double Scale =
           p.vector_tol ? (p.atol[i] + p.rtol[i] * Ymax) : (p.atol0 + p.rtol0 * Ymax);
       Err = Err + (ye * ye) / (Scale * Scale);
-    }
+    });
     Err = sqrt(Err / (double)NVAR);
     // step size control (:594-617)
     const double Fac = dmin(p.FacMax, dmax(p.FacMin, p.FacSafe / pow(Err, 1.0 / m.ELO)));
@@ -474,7 +587,7 @@ struct Cell {
       // step log: size of the first accepted step of this call
       if (p.hfirst && s.Nacc == p.stat0[3]) p.hfirst[s.cell] = H;
       s.Nacc++;
-      for (int i = 0; i < NVAR; i++) Y[i] = Ynew[i];
+      vloop<U, NVAR>([&](int i) { Y[i] = Ynew[i]; });
       s.T += Dir * H;
       Hnew = dmax(p.Hmin, dmin(Hnew, s.Hmax));
       if (s.RejectLastH) Hnew = dmin(Hnew, H);
@@ -523,6 +636,21 @@ __global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void ros_plain_kernel(RosPar
   const int k = blockIdx.x * p.lanes + threadIdx.x;
   if (k >= p.ncells) return;
   Cell<M, S, MIX> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + threadIdx.x};
+  CellState s;
+  c.start(p, p.order ? p.order[k] : k, s);
+  while (c.attempt(p, s) == 0) {
+  }
+}
+
+// Register kernel (small mechanisms, fits_registers<M>()): one lane per cell
+// for its whole integration like the plain kernel, with the working set in
+// the lane's VGPRs (RegMem) instead of a tile in global memory.
+template <class M, int S>
+__global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void ros_reg_kernel(RosParams p) {
+  if ((int)threadIdx.x >= p.lanes) return;
+  const int k = blockIdx.x * p.lanes + threadIdx.x;
+  if (k >= p.ncells) return;
+  Cell<M, S, false, RegMem<M, Layout<M, S, false>::ROWS>> c;
   CellState s;
   c.start(p, p.order ? p.order[k] : k, s);
   while (c.attempt(p, s) == 0) {

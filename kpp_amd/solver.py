@@ -27,6 +27,7 @@ SUN_FIXED = 0
 SUN_TOD = 1
 PLAIN = 0
 PERSISTENT = 1
+REGISTER = 2  # one lane per cell, working set in VGPRs (small mechanisms only)
 
 # IPAR[3] codes of int/rosenbrock.c:218-225
 ROS2, ROS3, ROS4, RODAS3, RODAS4 = 1, 2, 3, 4, 5

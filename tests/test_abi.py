@@ -85,3 +85,14 @@ def test_workspace_sizes():
         p = solver.workspace_bytes(m, meth, 100000, variant=solver.PERSISTENT, nblocks=64)
         assert 0 < p < solver.workspace_bytes(m, meth, 100000)
     assert solver.workspace_bytes("saprc99", 6, 10) == 0
+
+
+def test_register_variant_geometry():
+    # the register kernel needs no tiles: only the queue head and tolerances
+    r = solver.workspace_bytes("small_strato", solver.ROS3, 4096, variant=solver.REGISTER)
+    assert 0 < r <= 1024 < solver.workspace_bytes("small_strato", solver.ROS3, 4096)
+    # compiled for small mechanisms only: saprc99 (920 LU entries) is refused
+    assert solver.workspace_bytes("saprc99", solver.RODAS3, 4096, variant=solver.REGISTER) == 0
+    # and not with the FP32 factorisation
+    assert solver.workspace_bytes("small_strato", solver.ROS3, 4096, variant=solver.REGISTER,
+                                  lu_fp32=1) == 0

@@ -338,3 +338,35 @@ def test_context_api_and_integrate_batch(gpu):
         assert _rel({"C": got}, {"C": Cref}, o["nvar"]).max() <= 1e-8
     finally:
         L.kpp_ctx_destroy(ctx)
+
+
+# ------------------------------------------------------------ register kernel
+@pytest.mark.parametrize("method", [solver.ROS2, solver.ROS3, solver.ROS4, solver.RODAS3,
+                                    solver.RODAS4])
+def test_register_kernel_bit_identical_to_plain(gpu, method):
+    """small_strato with the working set in VGPRs: the same operations in the
+    same order as the tile kernels, so every cell's state, counters, Texit and
+    Hexit are bit for bit the plain kernel's (time-of-day SUN)."""
+    b = workloads.small_strato_tod(ncells=2000, method=method)  # not a multiple of 64
+    g0 = _gpu_run(b, gpu, variant=solver.PLAIN)
+    g1 = _gpu_run(b, gpu, variant=solver.REGISTER)
+    assert g1["code"] == 0 and (g1["ierr"] == g0["ierr"]).all()
+    assert np.array_equal(g1["C"], g0["C"])
+    assert (g1["istat"] == g0["istat"]).all()
+    assert np.array_equal(g1["rstat"], g0["rstat"])
+
+
+def test_register_kernel_matches_oracle(gpu):
+    b = workloads.small_strato_tod(ncells=4096)
+    o = ref.integrate(b)
+    g = _gpu_run(b, gpu, variant=solver.REGISTER, lanes=16)
+    assert (g["ierr"] == 1).all() and (o["ierr"] == 1).all()
+    assert (g["istat"][2] == o["istat"][2]).mean() >= 0.99
+    rel = _rel(g, o, o["nvar"])
+    assert rel.max() <= 1e-5 and np.median(rel) <= 1e-9
+
+
+def test_register_kernel_refused_for_large_mechanisms(gpu):
+    b = workloads.saprc99_uneven(ncells=64)
+    with pytest.raises(solver.KppError):
+        _gpu_run(b, gpu, variant=solver.REGISTER)
