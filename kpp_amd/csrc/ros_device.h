@@ -496,7 +496,8 @@ struct Cell {
         Mem::fun(Ynew, F, R, Fcn);
         s.Nfun++;
       }
-      // (registers: select the value, not the row, so every index stays static)
+      // the row holding this stage's function values (tiles; in registers the
+      // choice is made per value below, so every index stays static)
       A Fc = from_fcn0 ? Fcn0 : Fcn;
       double HC[S > 1 ? S - 1 : 1];
 #pragma unroll
