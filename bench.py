@@ -92,17 +92,23 @@ def cpu_baseline(wl_fn, ncells_full, seed, target_s=20.0):
     """Time the oracle on a bounded sample of the same workload."""
     from oracle import ref
     threads = len(os.sched_getaffinity(0))
-    cal = wl_fn(ncells=256, seed=seed)
+    ncal = max(1024, 128 * threads)
+    cal = wl_fn(ncells=ncal, seed=seed)
     t = time.time()
     ref.integrate(cal, nthreads=threads, fast=True)
-    rate = 256 / max(time.time() - t, 1e-3)
+    rate = ncal / max(time.time() - t, 1e-3)
     n = int(min(ncells_full, max(1024, rate * target_s)))
-    # an evenly spaced subset of the batch's cells
-    cells = np.linspace(0, ncells_full - 1, n).astype(np.int64)
-    b = wl_fn(seed=seed, cells=cells)
-    t = time.time()
-    out = ref.integrate(b, nthreads=threads, fast=True)
-    dt = time.time() - t
+    for _ in range(2):
+        # an evenly spaced subset of the batch's cells
+        cells = np.linspace(0, ncells_full - 1, n).astype(np.int64)
+        b = wl_fn(seed=seed, cells=cells)
+        t = time.time()
+        out = ref.integrate(b, nthreads=threads, fast=True)
+        dt = time.time() - t
+        # a calibration that overestimated the time per cell: size the sample again
+        if dt >= 0.4 * target_s or n >= ncells_full:
+            break
+        n = int(min(ncells_full, n * target_s / max(dt, 1e-3)))
     return {"value": n / dt, "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": f"{n} evenly spaced cells of the same {ncells_full}-cell batch, "
                       f"oracle ros_ref (gcc -O2 -march=x86-64-v3, OpenMP dynamic schedule), "
@@ -128,13 +134,13 @@ def copy_probe(nbytes=1 << 30, reps=3):
                       "reps": reps, "best_ms": ms, "hbm_gbs": 2 * nbytes / ms / 1e6}))
 
 
-def measured_traffic(workload, ncells):
+def measured_traffic(workload, ncells, kernel):
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("workload") != workload or t.get("ncells") != ncells:
+    if t.get("workload") != workload or t.get("ncells") != ncells or t.get("kernel") != kernel:
         return None
     return t.get("bytes_per_launch")
 
@@ -168,6 +174,8 @@ def main():
     ap.add_argument("--eager-tuned", action="store_true",
                     help="run the tuned path as eager calls instead of one graph (same kernels; "
                          "for per-dispatch hardware counters)")
+    ap.add_argument("--lds-entries", type=int, default=0,
+                    help="with --eager-tuned: LU entries per cell mirrored in LDS (0, 20, 24)")
     ap.add_argument("--lu-fp32", action="store_true",
                     help="also time the mixed-precision integrator and report it as value")
     ap.add_argument("--copy-probe", action="store_true",
@@ -214,8 +222,16 @@ def main():
                 "fresh batches (seeds 101, 102; untimed)")
             model, _ = learned.train_model(b.mech, wl_fn, [101, 102], args.ncells, dev, log=log)
 
+    # launch shape of the tuned path's eager calls: the autotuner's choice once
+    # the graph path has made it, else --lds-entries (for counter runs)
+    tuned_launch = {"lds_entries": args.lds_entries} if args.lds_entries else {}
+
     def step(variant, learned_path, lu_fp32=0):
         C.copy_(C0)
+        kw = {"nblocks": args.nblocks if variant == solver.PERSISTENT else 0,
+              "refill_min": args.refill_min}
+        if learned_path and variant == solver.PERSISTENT:
+            kw.update(tuned_launch)
         ev0, evk, ev1 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         ev0.record()
         hs = order = None
@@ -227,8 +243,7 @@ def main():
             b.mech, C, d["temp"], d["cfactor"], tstart=d["tstart"], tend=d["tend"],
             atol=b.atol, rtol=b.rtol, IPAR=b.IPAR, RPAR=b.RPAR, sun=d["sun"],
             sun_mode=b.sun_mode, hstart=hs, order=order, ierr=ierr, istat=istat, rstat=rstat,
-            variant=variant, nblocks=args.nblocks if variant == solver.PERSISTENT else 0,
-            refill_min=args.refill_min, lu_fp32=lu_fp32, fallbacks=fallbacks, workspace=ws)
+            variant=variant, lu_fp32=lu_fp32, fallbacks=fallbacks, workspace=ws, **kw)
         ev1.record()
         assert code == 0, code
         return ev0, evk, ev1
@@ -267,9 +282,11 @@ def main():
             b.mech, C, C0, d["temp"], d["cfactor"], tstart=d["tstart"], tend=d["tend"],
             atol=b.atol, rtol=b.rtol, IPAR=b.IPAR, RPAR=b.RPAR, sun=d["sun"],
             sun_mode=b.sun_mode, hstart=sched[0], order=sched[2])
-        launch = {"variant": best[0], "nblocks": best[1], "refill_min": best[2], "lanes": best[3]}
+        launch = {"variant": best[0], "nblocks": best[1], "refill_min": best[2], "lanes": best[3],
+                  "lds_entries": best[4]}
         log(f"[bench] autotuner: {tried} launch shapes in {time.time() - t:.1f}s, best "
             f"{launch} at {best_ms:.1f} ms")
+        tuned_launch.update({k: v for k, v in launch.items() if k != "variant"})
 
         def body():
             model.features(C, d["temp"], d["sun"], out=X)
@@ -344,6 +361,9 @@ def main():
 
     # roofline of the persistent kernel on the tuned run's attempts
     top_mixed = (model is None and args.lu_fp32)
+    top_lds = tuned_launch.get("lds_entries", 0) if model is not None else 0
+    top_kernel = (f"ros_persistent_kernel<kppm_{b.mech},{solver.METHOD_STAGES[b.method]}"
+                  f"{',MIX' if top_mixed else ''}{',LDS' + str(top_lds) if top_lds else ''}>")
     nbytes, nflops, attempts, bterms = algorithmic_work(mech, b.method, top["istat"], top_mixed)
     kern_t = top["kernel_s"]
 
@@ -448,9 +468,8 @@ def main():
                    "all_ierr_1": top["ok"], "attempts_per_cell": attempts / n},
         "roofline": roof or {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, n),
-                     "kernel": f"ros_persistent_kernel<kppm_{b.mech},"
-                               f"{solver.METHOD_STAGES[b.method]}{',MIX' if top_mixed else ''}>",
+                     "traffic": measured_traffic(args.workload, n, top_kernel),
+                     "kernel": top_kernel,
                      "kernel_ms": kern_t * 1e3, "algorithmic_bytes": nbytes,
                      "fp64_tflops": nflops / kern_t / 1e12,
                      "fp64_frac": nflops / kern_t / 1e12 / FP64_PEAK_TFLOPS, **bterms},

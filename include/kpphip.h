@@ -106,6 +106,11 @@ typedef struct {
                      (correction above 1e-5 of the solution, or not finite) finishes the
                      attempt with the FP64 factorisation.  State, stage sums, error norm
                      and step control stay FP64. */
+  int lds_entries; /* persistent, FP64 only: the factored values of this many LU entries
+                      per cell (those KppDecomp re-reads most) are kept in LDS instead of
+                      the workspace tile; no counterpart in KPP, results bitwise the same.
+                      0 (none), 20 or 24; 20 and 24 only for mechanisms built with an LDS
+                      mirror (saprc99, saprcnov), anything else is KPP_E_ARG. */
 } kpp_launch;
 
 /* Device bytes of workspace needed for a launch. */

@@ -36,10 +36,10 @@ ORACLE_BUILD = ORACLE / "build"
 # (paths in them are relative to KPP_HOME too)
 MECHANISMS = {
     "small_strato": ("models/small_strato.def", []),
-    "saprc99": ("models/saprc99.def", []),
+    "saprc99": ("models/saprc99.def", ["--lds-split", "24"]),
     # KPP's second SAPRC-99 model (models/saprcnov.*): same rate-law syntax, 88
     # variable species, N2 among the fixed species
-    "saprcnov": ("models/saprcnov.def", []),
+    "saprcnov": ("models/saprcnov.def", ["--lds-split", "24"]),
     # MCM isoprene export: rate code in Fortran 90 (examples/mcm/constants_mcm.f90,
     # inlined through #INLINE F90_RCONST), with the air density and its O2, N2,
     # H2O fractions and the solar zenith angle from examples/mcm/driver_mcm.f90

@@ -1484,6 +1484,7 @@ struct PipeParams {
 };
 PipeParams g_pipe;
 bool g_emit_pipelined = true;
+int g_lds_split = 0;   // LU entries the KppDecompL/KppSolveL variants may mirror in LDS (0: no variants)
 long g_part_ops = 0;  // split routines into out-of-line parts of this many operations (0: no)
 
 struct PipeOpd {
@@ -1622,10 +1623,27 @@ void emit_hip(Model& md, const std::string& path) {
     << "// HIP device functions of the model; arithmetic and association order are\n"
     << "// those of KPP's C output, so with FP contraction off the results are\n"
     << "// bitwise those of the C model.\n#pragma once\n#include <hip/hip_runtime.h>\n\n";
+  if (g_lds_split > 0)
+    o << "#ifndef KPPHIP_LDS_SPLIT_HELPERS\n#define KPPHIP_LDS_SPLIT_HELPERS\n"
+      << "// Entry x of the LU with mirror slot R: in the LDS mirror Gs when R < NL,\n"
+      << "// else in G.  R and NL are compile-time constants, so each access is one\n"
+      << "// or the other.\n"
+      << "template <int NL, int R, class TL, class TG>\n"
+      << "static __device__ __forceinline__ typename TG::value_type kpp_lget(TL Gs, TG G, int x) {\n"
+      << "  if constexpr (R < NL) return Gs[R]; else return G[x];\n}\n"
+      << "template <int NL, int R, class TL, class TG>\n"
+      << "static __device__ __forceinline__ void kpp_lput(TL Gs, TG G, int x,\n"
+      << "                                                typename TG::value_type v) {\n"
+      << "  if constexpr (R < NL) Gs[R] = v; else G[x] = v;\n}\n"
+      << "template <int NL, int R, class TL, class TG>\n"
+      << "static __device__ __forceinline__ void kpp_lcopy(TL Gs, TG G, int x) {\n"
+      << "  if constexpr (R < NL) Gs[R] = G[x];\n}\n#endif\n\n";
   o << "struct " << ns << " {\n";
   o << "static constexpr int NVAR = " << S.nvar << ", NFIX = " << S.nfix << ", NSPEC = " << S.nspc
     << ", NREACT = " << S.nreact << ", NONZERO = " << S.nonzero
     << ", LU_NONZERO = " << S.lu_nonzero << ";\n";
+  // LU entries KppDecompL/KppSolveL can mirror in LDS (slots 0..LDS_MAX-1)
+  o << "static constexpr int LDS_MAX = " << std::min(g_lds_split, S.lu_nonzero) << ";\n";
   o << "static constexpr const char* ROOT = \"" << m.root << "\";\n";
   // rate dependence classes
   bool any_time = false;
@@ -1714,7 +1732,8 @@ void emit_hip(Model& md, const std::string& path) {
   // parts' results combine as the smallest nonzero value.
   auto routine = [&](const std::string& tparams, const std::string& name,
                      const std::string& params, const std::string& args, int kind,
-                     const std::vector<std::pair<std::string, long>>& stmts) {
+                     const std::vector<std::pair<std::string, long>>& stmts,
+                     const std::string& targs = "") {
     const char* rt = kind ? "int" : "void";
     std::vector<std::string> parts(1);
     long ops = 0;
@@ -1741,10 +1760,11 @@ void emit_hip(Model& md, const std::string& path) {
     if (kind == 1) o << "  int r = 0;\n";
     if (kind == 2) o << "  int ising = 0, r = 0;\n";
     for (size_t q = 0; q < parts.size(); q++) {
-      if (kind == 0) o << "  " << name << "_p" << q << "(" << args << ");\n";
-      if (kind == 1) o << "  if ((r = " << name << "_p" << q << "(" << args << "))) return r;\n";
+      if (kind == 0) o << "  " << name << "_p" << q << targs << "(" << args << ");\n";
+      if (kind == 1)
+        o << "  if ((r = " << name << "_p" << q << targs << "(" << args << "))) return r;\n";
       if (kind == 2)
-        o << "  r = " << name << "_p" << q << "(" << args << ");\n"
+        o << "  r = " << name << "_p" << q << targs << "(" << args << ");\n"
           << "  if (r && (ising == 0 || r < ising)) ising = r;\n";
     }
     o << post << "}\n";
@@ -1858,6 +1878,87 @@ void emit_hip(Model& md, const std::string& path) {
     routine("template <class TG>", checked ? "KppDecompChecked" : "KppDecomp", "TG G", "G",
             checked ? 1 : 0, st);
   }
+  // LDS-split variants (--lds-split N).  KppDecompL<NL>(G, Gs) and
+  // KppSolveL<NL>(G, X, Gs) are KppDecomp and KppSolve with the factored
+  // values of the NL entries KppDecomp re-reads most (pivot-row reads; slot 0
+  // the most read) kept in Gs instead of G: the row-end store goes to Gs and
+  // every later read of the entry (elimination of the rows below, the
+  // triangular solves) comes from Gs.  G keeps the unfactored values of those
+  // entries.  Same operations in the same order, so the results are bitwise
+  // those of KppDecomp/KppSolve for every NL <= LDS_MAX.
+  std::vector<int> lds_slot(S.lu_nonzero, -1);
+  if (g_lds_split > 0) {
+    std::vector<long> reads(S.lu_nonzero, 0);
+    for (int k = 0; k < S.nvar; k++)
+      for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
+        const int j = S.lu_icol[kk];
+        reads[S.lu_diag[j]]++;
+        for (int jj = S.lu_diag[j] + 1; jj < S.lu_crow[j + 1]; jj++) reads[jj]++;
+      }
+    std::vector<int> by(S.lu_nonzero);
+    for (int x = 0; x < S.lu_nonzero; x++) by[x] = x;
+    std::stable_sort(by.begin(), by.end(), [&](int a, int b) { return reads[a] > reads[b]; });
+    const int nl = std::min(g_lds_split, S.lu_nonzero);
+    for (int r = 0; r < nl; r++) lds_slot[by[r]] = r;
+  }
+  auto lget = [&](const char* arr, int x) {
+    if (lds_slot[x] < 0) return std::string(arr) + "[" + std::to_string(x) + "]";
+    return "kpp_lget<NL, " + std::to_string(lds_slot[x]) + ">(Gs, " + arr + ", " +
+           std::to_string(x) + ")";
+  };
+  if (g_lds_split > 0) {
+    std::vector<std::pair<std::string, long>> st;
+    for (int k = 0; k < S.nvar; k++) {
+      std::string t;
+      long n_ops = 0;
+      std::map<int, int> pos;
+      for (int kk = S.lu_crow[k]; kk < S.lu_crow[k + 1]; kk++) pos[S.lu_icol[kk]] = kk;
+      std::vector<int> touched;
+      std::string ops;
+      for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
+        const int j = S.lu_icol[kk];
+        touched.push_back(kk);
+        ops += "    w" + std::to_string(kk) + " = w" + std::to_string(kk) + "/" +
+               lget("G", S.lu_diag[j]) + ";";
+        n_ops++;
+        for (int jj = S.lu_diag[j] + 1; jj < S.lu_crow[j + 1]; jj++) {
+          auto it = pos.find(S.lu_icol[jj]);
+          if (it == pos.end()) die("LU pattern not closed under elimination");
+          const std::string w = "w" + std::to_string(it->second);
+          touched.push_back(it->second);
+          ops += " " + w + " = " + w + " - w" + std::to_string(kk) + "*" + lget("G", jj) + ";";
+          n_ops += 2;
+        }
+        ops += "\n";
+      }
+      std::sort(touched.begin(), touched.end());
+      touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+      if (!touched.empty()) {
+        t += "  {\n   ";
+        for (int x : touched)
+          t += " typename TG::value_type w" + std::to_string(x) + " = G[" + std::to_string(x) + "];";
+        t += "\n" + ops + "   ";
+        for (int x : touched) {
+          const std::string w = "w" + std::to_string(x);
+          if (lds_slot[x] < 0)
+            t += " G[" + std::to_string(x) + "] = " + w + ";";
+          else
+            t += " kpp_lput<NL, " + std::to_string(lds_slot[x]) + ">(Gs, G, " +
+                 std::to_string(x) + ", " + w + ");";
+        }
+        t += "\n  }\n";
+      }
+      // entries of row k the elimination leaves as they are: their mirror
+      // slot gets the value from G
+      for (int kk = S.lu_crow[k]; kk < S.lu_crow[k + 1]; kk++)
+        if (lds_slot[kk] >= 0 && !std::binary_search(touched.begin(), touched.end(), kk))
+          t += "  kpp_lcopy<NL, " + std::to_string(lds_slot[kk]) + ">(Gs, G, " +
+               std::to_string(kk) + ");\n";
+      st.push_back({t, n_ops});
+    }
+    routine("template <int NL, class TG, class TL>", "KppDecompL", "TG G, TL Gs", "G, Gs", 0, st,
+            "<NL>");
+  }
   if (g_emit_pipelined) emit_pipelined(o, S, g_pipe);
   // Residual of the unfactored matrix: R = B - G*X over the entries of the
   // pattern that can be nonzero (Jacobian entries and the diagonal; fill-in
@@ -1877,14 +1978,17 @@ void emit_hip(Model& md, const std::string& path) {
     routine("template <class TG, class TX, class TB, class TR>", "Residual",
             "TG G, TX X, TB B, TR R", "G, X, B, R", 0, st);
   }
-  {
+  for (int split = 0; split < (g_lds_split > 0 ? 2 : 1); split++) {
+    auto jvs = [&](int x) {
+      return split ? lget("JVS", x) : "JVS[" + std::to_string(x) + "]";
+    };
     std::vector<std::pair<std::string, long>> st;
     for (int i = 0; i < S.nvar; i++) {
       int b = S.lu_crow[i], e = S.lu_diag[i];
       if (b < e) {
         std::string t = "  X[" + std::to_string(i) + "] = X[" + std::to_string(i) + "]";
         for (int j = b; j < e; j++)
-          t += "-JVS[" + std::to_string(j) + "]*X[" + std::to_string(S.lu_icol[j]) + "]";
+          t += "-" + jvs(j) + "*X[" + std::to_string(S.lu_icol[j]) + "]";
         st.push_back({t + ";\n", 2L * (e - b)});
       }
     }
@@ -1894,15 +1998,19 @@ void emit_hip(Model& md, const std::string& path) {
       if (b < e) {
         t = "  X[" + std::to_string(i) + "] = (X[" + std::to_string(i) + "]";
         for (int j = b; j < e; j++)
-          t += "-JVS[" + std::to_string(j) + "]*X[" + std::to_string(S.lu_icol[j]) + "]";
-        t += ")/JVS[" + std::to_string(S.lu_diag[i]) + "];\n";
+          t += "-" + jvs(j) + "*X[" + std::to_string(S.lu_icol[j]) + "]";
+        t += ")/" + jvs(S.lu_diag[i]) + ";\n";
       } else {
-        t = "  X[" + std::to_string(i) + "] = X[" + std::to_string(i) + "]/JVS[" +
-            std::to_string(S.lu_diag[i]) + "];\n";
+        t = "  X[" + std::to_string(i) + "] = X[" + std::to_string(i) + "]/" +
+            jvs(S.lu_diag[i]) + ";\n";
       }
       st.push_back({t, 2L * (e - b) + 1});
     }
-    routine("template <class TG, class TX>", "KppSolve", "TG JVS, TX X", "JVS, X", 0, st);
+    if (split)
+      routine("template <int NL, class TG, class TX, class TL>", "KppSolveL",
+              "TG JVS, TX X, TL Gs", "JVS, X, Gs", 0, st, "<NL>");
+    else
+      routine("template <class TG, class TX>", "KppSolve", "TG JVS, TX X", "JVS, X", 0, st);
   }
   o << "#undef ARR_abc\n#undef ARR_ab\n#undef ARR_ac\n#undef EP2\n#undef EP3\n#undef FALL\n";
   o << "};  // struct " << ns << "\n";
@@ -1971,6 +2079,7 @@ int main(int argc, char** argv) {
     else if (a == "--rates-f90") rates_f90 = true;
     else if (a == "--no-pipelined") g_emit_pipelined = false;
     else if (a == "--part-ops") g_part_ops = atol(next().c_str());
+    else if (a == "--lds-split") g_lds_split = atoi(next().c_str());
     else if (a == "--f90-module") f90_modules.push_back(next());
     else if (a == "--env") env = next();
     else if (a == "--pf-dist") g_pipe.dist = atoi(next().c_str());
@@ -1979,6 +2088,7 @@ int main(int argc, char** argv) {
     else if (a == "-h" || a == "--help") {
       printf("usage: kppgen [--kpp-home DIR] [--root NAME] [-o OUTDIR] [--rates-f90\n"
              "              [--f90-module FILE]... [--env 'F90 statements;...']]\n"
+             "              [--part-ops N] [--lds-split N] [--no-pipelined]\n"
              "              mechanism.{def,kpp}\n");
       return 0;
     } else in = a;

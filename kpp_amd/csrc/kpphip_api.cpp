@@ -186,6 +186,7 @@ struct Geometry {
   int nblocks;
   int refill_min;
   int lanes;
+  int lds_entries;
   size_t tol_bytes, tile_bytes, rpre_bytes, total;
 };
 
@@ -201,9 +202,17 @@ bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry
     return false;
   if (variant == KPP_VARIANT_REGISTER && (mixed || !m->fits_registers)) return false;
   if (l && l->lu_fp32 != 0 && l->lu_fp32 != 1) return false;
+  // LDS mirror of LU entries: persistent FP64 kernel, sizes the mechanism is built with
+  const int lds = l ? l->lds_entries : 0;
+  if (lds != 0) {
+    if (variant != KPP_VARIANT_PERSISTENT || mixed || lds > m->lds_max ||
+        (lds != KPP_LDS_SPLIT_A && lds != KPP_LDS_SPLIT_B))
+      return false;
+  }
+  g.lds_entries = lds;
   int cus = device_cus();
   if (cus <= 0) cus = 256;
-  int occ = mixed ? mx->occupancy(variant, S) : m->occupancy(variant, S);
+  int occ = mixed ? mx->occupancy(variant, S, 0) : m->occupancy(variant, S, lds);
   if (occ <= 0) occ = 4;
   const int resident = cus * occ;  // wavefronts the device holds at once
   int lanes = (l && l->lanes > 0) ? l->lanes : 0;
@@ -404,6 +413,7 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
   p.scratch = (double*)(ws + kHead + g.tol_bytes + g.rpre_bytes);
   p.refill_min = g.refill_min;
   p.lanes = g.lanes;
+  p.lds_entries = g.lds_entries;
   const int variant = l ? l->variant : KPP_VARIANT_PLAIN;
   const hipError_t le = (l && l->lu_fp32) ? mixed_of(m)->launch(p, variant, g.nblocks, st)
                                           : m->launch(p, variant, g.nblocks, st);
@@ -492,7 +502,7 @@ int kpp_ctx_create(kpp_ctx** out, const char* mech, int ncells, int method, int 
   c->rtol.assign(1, 1.0e-3);
   c->vector_tol = 0;
   const size_t n = (size_t)ncells;
-  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0, 0};
+  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0, 0, 0};
   c->ws_bytes = kpp_workspace_bytes(mech, method, &l, ncells);
   if (c->ws_bytes == 0) { ctx_free(c); return -2; }
   bool ok = hipMalloc(&c->C, sizeof(double) * n * m->nspec) == hipSuccess &&
@@ -549,7 +559,7 @@ static int ctx_run(kpp_ctx* c, const int* IPAR, const double* RPAR, const double
   b.ierr = c->ierr;
   b.istat = c->istat;
   b.rstat = c->rstat;
-  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0, 0};
+  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0, 0, 0};
   int r = kpp_rosenbrock_dev(c->m->name, &b, IPAR, RPAR, atol, rtol, &l, c->ws, c->ws_bytes,
                              nullptr);
   if (hipDeviceSynchronize() != hipSuccess) return KPP_E_HIP;
@@ -668,13 +678,17 @@ extern "C" void kpp_graph_destroy(kpp_graph* g) {
 
 // ---- autotuner ----
 // Candidate launch shapes of the persistent kernel: cells per wavefront
-// (lanes), refill threshold, and grid size relative to the resident capacity.
+// (lanes), refill threshold, grid size relative to the resident capacity,
+// and the LU entries per lane mirrored in LDS (candidates with a mirror the
+// mechanism is not built with are skipped).
 namespace {
 struct Cand {
-  int lanes, refill_min, grid_pct;
+  int lanes, refill_min, grid_pct, lds;
 };
-const Cand kCands[] = {{64, 1, 100}, {64, 8, 100}, {64, 1, 75},  {64, 1, 125},
-                       {32, 1, 100}, {32, 4, 100}, {48, 1, 100}, {64, 16, 100}};
+const Cand kCands[] = {{64, 1, 100, 0},  {64, 8, 100, 0},  {64, 1, 75, 0},
+                       {64, 1, 125, 0},  {32, 1, 100, 0},  {32, 4, 100, 0},
+                       {48, 1, 100, 0},  {64, 16, 100, 0}, {64, 1, 100, KPP_LDS_SPLIT_A},
+                       {64, 1, 100, KPP_LDS_SPLIT_B}, {64, 1, 125, KPP_LDS_SPLIT_B}};
 constexpr int kNCand = sizeof(kCands) / sizeof(kCands[0]);
 
 bool cand_launch(const MechOps* m, int S, int ncells, const Cand& c, kpp_launch& l) {
@@ -683,6 +697,8 @@ bool cand_launch(const MechOps* m, int S, int ncells, const Cand& c, kpp_launch&
   l.lanes = c.lanes;
   l.refill_min = c.refill_min;
   l.nblocks = 0;
+  l.lds_entries = c.lds;
+  if (c.lds > m->lds_max) return false;
   Geometry g;
   if (!geometry(m, S, &l, ncells, g)) return false;
   l.nblocks = std::max(1, (int)((long)g.nblocks * c.grid_pct / 100));

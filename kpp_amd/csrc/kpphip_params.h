@@ -54,6 +54,7 @@ struct RosParams {
   int* queue;              // persistent kernel: next-cell counter (starts at 0)
   int refill_min;          // persistent kernel: refill when >= this many lanes idle
   int lanes;               // cells per wavefront (1..64)
+  int lds_entries;         // persistent kernel: LU entries per lane mirrored in LDS (0: none)
   RosCoef coef;
 };
 

@@ -69,13 +69,52 @@ __global__ __launch_bounds__(256) void k_rates(int n, size_t ld, const double* t
 
 static __global__ void k_zero_int(int* q) { *q = 0; }
 
+// Persistent kernel with an LDS mirror of NL LU entries per lane (NL > 0:
+// FP64 only, and only if the mechanism is built with a mirror that large);
+// NL == 0 is the plain tile layout, FP64 or mixed.
+template <class M, bool MIX, int NL>
+constexpr bool has_lds_split() {
+  return NL == 0 || (!MIX && NL <= M::LDS_MAX);
+}
+template <class M, int S, bool MIX, int NL>
+hipError_t launch_persistent(const RosParams& p, int nblocks, hipStream_t st) {
+  if constexpr (has_lds_split<M, MIX, NL>()) {
+    hipLaunchKernelGGL((ros_persistent_kernel<M, S, MIX, NL>), dim3(nblocks), dim3(TILE),
+                       (size_t)NL * TILE * sizeof(double), st, p);
+    return hipSuccess;
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+template <class M, int S, bool MIX, int NL>
+int occupancy_persistent() {
+  int n = 0;
+  if constexpr (has_lds_split<M, MIX, NL>()) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_persistent_kernel<M, S, MIX, NL>,
+                                                     TILE, (size_t)NL * TILE * sizeof(double)) !=
+        hipSuccess)
+      n = 0;
+  }
+  return n;
+}
+
 template <class M, int S, bool MIX>
 hipError_t launch_s(const RosParams& p, int variant, int nblocks, hipStream_t st) {
   if (variant == KPP_VARIANT_PERSISTENT) {
+    if (p.lds_entries != 0 && p.lds_entries != KPP_LDS_SPLIT_A &&
+        p.lds_entries != KPP_LDS_SPLIT_B)
+      return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, p.queue);
     if (p.rpre)
       hipLaunchKernelGGL(ros_rates_prepass<M>, dim3((p.ncells + 255) / 256), dim3(256), 0, st, p);
-    hipLaunchKernelGGL((ros_persistent_kernel<M, S, MIX>), dim3(nblocks), dim3(TILE), 0, st, p);
+    hipError_t e;
+    if (p.lds_entries == KPP_LDS_SPLIT_A)
+      e = launch_persistent<M, S, MIX, KPP_LDS_SPLIT_A>(p, nblocks, st);
+    else if (p.lds_entries == KPP_LDS_SPLIT_B)
+      e = launch_persistent<M, S, MIX, KPP_LDS_SPLIT_B>(p, nblocks, st);
+    else
+      e = launch_persistent<M, S, MIX, 0>(p, nblocks, st);
+    if (e != hipSuccess) return e;
   } else if (variant == KPP_VARIANT_REGISTER) {
     if constexpr (fits_registers<M>() && !MIX)
       hipLaunchKernelGGL((ros_reg_kernel<M, S>), dim3(nblocks), dim3(TILE), 0, st, p);
@@ -100,11 +139,14 @@ hipError_t launch_mech(const RosParams& p, int variant, int nblocks, hipStream_t
 
 // Resident wavefronts per CU of a kernel (its register/LDS limits).
 template <class M, int S, bool MIX>
-int occupancy_s(int variant) {
+int occupancy_s(int variant, int lds) {
   int n = 0;
   hipError_t e;
   if (variant == KPP_VARIANT_PERSISTENT) {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_persistent_kernel<M, S, MIX>, TILE, 0);
+    if (lds == KPP_LDS_SPLIT_A) return occupancy_persistent<M, S, MIX, KPP_LDS_SPLIT_A>();
+    if (lds == KPP_LDS_SPLIT_B) return occupancy_persistent<M, S, MIX, KPP_LDS_SPLIT_B>();
+    if (lds != 0) return 0;
+    return occupancy_persistent<M, S, MIX, 0>();
   } else if (variant == KPP_VARIANT_REGISTER) {
     if constexpr (fits_registers<M>() && !MIX)
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ros_reg_kernel<M, S>, TILE, 0);
@@ -116,12 +158,12 @@ int occupancy_s(int variant) {
   return e == hipSuccess ? n : 0;
 }
 template <class M, bool MIX>
-int occupancy_mech(int variant, int S) {
+int occupancy_mech(int variant, int S, int lds) {
   switch (S) {
-    case 2: return occupancy_s<M, 2, MIX>(variant);
-    case 3: return occupancy_s<M, 3, MIX>(variant);
-    case 4: return occupancy_s<M, 4, MIX>(variant);
-    case 6: return occupancy_s<M, 6, MIX>(variant);
+    case 2: return occupancy_s<M, 2, MIX>(variant, lds);
+    case 3: return occupancy_s<M, 3, MIX>(variant, lds);
+    case 4: return occupancy_s<M, 4, MIX>(variant, lds);
+    case 6: return occupancy_s<M, 6, MIX>(variant, lds);
   }
   return 0;
 }
@@ -171,6 +213,8 @@ MechOps make_ops(const char* name) {
   o.rates_use_state = M::RATES_USE_STATE ? 1 : 0;
   o.n_state_rates = M::N_STATE_RATES;
   o.fits_registers = fits_registers<M>() ? 1 : 0;
+  o.lds_max = M::LDS_MAX >= KPP_LDS_SPLIT_B ? KPP_LDS_SPLIT_B
+            : (M::LDS_MAX >= KPP_LDS_SPLIT_A ? KPP_LDS_SPLIT_A : 0);
   o.species = D::names();
   o.lu_icol = D::icol();
   o.lu_crow = D::crow();

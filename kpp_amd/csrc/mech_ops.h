@@ -9,6 +9,14 @@
 #define KPP_VARIANT_PERSISTENT 1
 #define KPP_VARIANT_REGISTER 2
 
+// LDS mirror sizes (LU entries per lane) the persistent kernel is built with,
+// where the mechanism's LDS_MAX allows (24 x 64 lanes x 8 B = 12 KiB per
+// one-wavefront workgroup).  Larger mirrors fit in LDS but were slower on
+// saprc99: the mirrored values held in registers through the triangular
+// solves push the compiler into a one-load-at-a-time schedule there.
+#define KPP_LDS_SPLIT_A 20
+#define KPP_LDS_SPLIT_B 24
+
 #define KPP_COMP_FUN 0
 #define KPP_COMP_JAC 1
 #define KPP_COMP_DECOMP 2
@@ -22,11 +30,13 @@ struct MechOps {
   int nvar, nfix, nspec, nreact, nonzero, lu_nonzero;
   int rates_use_time, n_sun_rates, rates_use_state, n_state_rates;
   int fits_registers;  // KPP_VARIANT_REGISTER is compiled for this mechanism
+  int lds_max;         // largest LDS mirror (LU entries) the persistent kernel is built with
   const char* const* species;
   const int *lu_icol, *lu_crow, *lu_diag;
   const double* init;  // Initialize(): #INITVALUES times CFACTOR, [NSPEC]
   int (*rows)(int S);  // tile rows per lane for S stages
-  int (*occupancy)(int variant, int S);  // resident wavefronts per CU (0 if unknown)
+  // resident wavefronts per CU (0 if unknown) with `lds` mirrored LU entries per lane
+  int (*occupancy)(int variant, int S, int lds);
   hipError_t (*launch)(const RosParams& p, int variant, int nblocks, hipStream_t st);
 };
 
@@ -39,7 +49,7 @@ typedef hipError_t (*ComponentFn)(int which, int n, size_t ld, const double* a, 
 // FP32 LU and triangular solves with one FP64 refinement sweep per solve.
 struct MixedOps {
   int (*rows)(int S);
-  int (*occupancy)(int variant, int S);
+  int (*occupancy)(int variant, int S, int lds);
   hipError_t (*launch)(const RosParams& p, int variant, int nblocks, hipStream_t st);
 };
 

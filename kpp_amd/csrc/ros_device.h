@@ -81,6 +81,14 @@ struct Acc {
   gdouble* __restrict__ b;
   __device__ __forceinline__ gdouble& operator[](int i) const { return b[i * TILE]; }
 };
+// One lane's LDS mirror of LU entries (KppDecompL/KppSolveL): slot r at
+// b[r*TILE], so the 64 lanes of a wavefront read 512 consecutive bytes.
+using ldouble = __attribute__((address_space(3))) double;
+struct AccL {
+  using value_type = double;
+  ldouble* b;
+  __device__ __forceinline__ ldouble& operator[](int i) const { return b[i * TILE]; }
+};
 using gfloat = __attribute__((address_space(1))) float;
 struct AccF {
   using value_type = float;
@@ -173,6 +181,15 @@ KPPHIP_MODEL_CALL void call_solve(dptr G, dptr X) {
   M::KppSolve(Acc{G}, Acc{X});
 #endif
 }
+// LU with the first NL mirror slots in LDS (NL <= M::LDS_MAX)
+template <class M, int NL>
+KPPHIP_MODEL_CALL void call_decomp_l(dptr G, ldouble* Gs) {
+  M::template KppDecompL<NL>(Acc{G}, AccL{Gs});
+}
+template <class M, int NL>
+KPPHIP_MODEL_CALL void call_solve_l(dptr G, dptr X, ldouble* Gs) {
+  M::template KppSolveL<NL>(Acc{G}, Acc{X}, AccL{Gs});
+}
 using fptr = gfloat* __restrict__;
 template <class M>
 KPPHIP_MODEL_CALL void call_decomp32(fptr G) {
@@ -204,12 +221,15 @@ KPPHIP_MODEL_CALL void call_rates_fixed(double TEMP, double CFACTOR, dptr R) {
 // Storage of a lane's working set.
 //
 // TileMem: one column of a 64-lane tile in global memory (every mechanism);
-// the model routines are called out of line on row pointers.
-template <class M>
+// the model routines are called out of line on row pointers.  With NL > 0
+// the factored values of the NL most re-read LU entries live in the lane's
+// LDS mirror `lds` instead (persistent kernel).
+template <class M, int NL = 0>
 struct TileMem {
   using A = Acc;
   static constexpr bool kRegisters = false;
   double* base;  // lane base pointer inside its tile
+  ldouble* lds;  // NL > 0: the lane's mirror slots (slot r at lds[r*TILE])
   __device__ __forceinline__ A at(int row) const { return Acc{(gdouble*)(base + row * TILE)}; }
   __device__ __forceinline__ static void fun(A V, A F, A R, A out) {
     call_fun<M>(V.b, F.b, R.b, out.b);
@@ -217,8 +237,18 @@ struct TileMem {
   __device__ __forceinline__ static int jac_ghimj(A V, A F, A R, double ghinv, A G) {
     return call_jac_ghimj<M>(V.b, F.b, R.b, ghinv, G.b);
   }
-  __device__ __forceinline__ static void decomp(A G) { call_decomp<M>(G.b); }
-  __device__ __forceinline__ static void solve(A G, A X) { call_solve<M>(G.b, X.b); }
+  __device__ __forceinline__ void decomp(A G) const {
+    if constexpr (NL > 0)
+      call_decomp_l<M, NL>(G.b, lds);
+    else
+      call_decomp<M>(G.b);
+  }
+  __device__ __forceinline__ void solve(A G, A X) const {
+    if constexpr (NL > 0)
+      call_solve_l<M, NL>(G.b, X.b, lds);
+    else
+      call_solve<M>(G.b, X.b);
+  }
   __device__ __forceinline__ static void rates_sun(double T, double CF, double SUN, double TIME,
                                                    A R) {
     call_rates_sun<M>(T, CF, SUN, TIME, R.b);
@@ -468,7 +498,7 @@ struct Cell {
       for (int i = 0; i < M::LU_NONZERO; i++) G32[i] = (float)G[i];
       call_decomp32<M>(G32.b);
     } else {
-      Mem::decomp(G);
+      mem.decomp(G);
     }
 
     const double H = s.H;
@@ -549,14 +579,14 @@ struct Cell {
           // this and the remaining stages use the FP64 factorisation
           lu64 = true;
           s.Nfb++;
-          Mem::decomp(G);
-          Mem::solve(G, Ki);
+          mem.decomp(G);
+          mem.solve(G, Ki);
         }
        } else {
-        Mem::solve(G, Ki);
+        mem.solve(G, Ki);
        }
       } else {
-        Mem::solve(G, Ki);
+        mem.solve(G, Ki);
       }
       s.Nsol++;
     };
@@ -665,11 +695,18 @@ __global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void ros_reg_kernel(RosParam
 // single atomic and the idle lanes take consecutive entries.  A wavefront
 // leaves once the queue is exhausted and none of its lanes is active, so
 // every wavefront reaches the exit.  Lanes at or above p.lanes never work.
-template <class M, int S, bool MIX = false>
+//
+// NL > 0 (FP64 only): the factored values of the NL most re-read LU entries
+// of each lane are kept in LDS (dynamic shared memory of NL*TILE doubles per
+// one-wavefront workgroup) instead of the tile; see KppDecompL.
+template <class M, int S, bool MIX = false, int NL = 0>
 __global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void ros_persistent_kernel(RosParams p) {
+  static_assert(NL == 0 || (!MIX && NL <= M::LDS_MAX), "LDS mirror: FP64, NL <= LDS_MAX");
   using L = Layout<M, S, MIX>;
   const int lane = threadIdx.x;
-  Cell<M, S, MIX> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + lane};
+  extern __shared__ double kpp_lds_mirror[];
+  Cell<M, S, MIX, TileMem<M, NL>> c{{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + lane,
+                                     (ldouble*)(kpp_lds_mirror + lane)}};
   CellState s;
   const bool worker = lane < p.lanes;
   bool active = false;

@@ -66,7 +66,8 @@ class _Batch(ctypes.Structure):
 
 class _Launch(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("nblocks", ctypes.c_int), ("refill_min", ctypes.c_int),
-                ("lanes", ctypes.c_int), ("lu_fp32", ctypes.c_int)]
+                ("lanes", ctypes.c_int), ("lu_fp32", ctypes.c_int),
+                ("lds_entries", ctypes.c_int)]
 
 
 _lib = None
@@ -209,15 +210,15 @@ _default_ws = {}
 
 
 def workspace_bytes(mech: str, method: int, ncells: int, variant=PLAIN, nblocks=0,
-                    refill_min=0, lanes=0, lu_fp32=0) -> int:
-    l = _Launch(variant, nblocks, refill_min, lanes, lu_fp32)
+                    refill_min=0, lanes=0, lu_fp32=0, lds_entries=0) -> int:
+    l = _Launch(variant, nblocks, refill_min, lanes, lu_fp32, lds_entries)
     return int(library().kpp_workspace_bytes(mech.encode(), method, ctypes.byref(l), ncells))
 
 
 def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, RPAR=None,
                sun=None, sun_mode=SUN_FIXED, hstart=None, rconst=None, order=None,
                hfirst=None, ierr=None, istat=None, rstat=None, variant=PLAIN, nblocks=0, refill_min=0,
-               lanes=0, lu_fp32=0, fallbacks=None, workspace=None, stream=None):
+               lanes=0, lu_fp32=0, lds_entries=0, fallbacks=None, workspace=None, stream=None):
     """Batched Rosenbrock on device tensors (float64 SoA, C is [NSPEC, ncells]).
 
     tstart/tend: per-cell tensors or floats.  atol/rtol: host sequences of
@@ -227,7 +228,8 @@ def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, R
     [ncells], optional) receives each cell's first accepted step size.
     ``lu_fp32=1`` selects the mixed-precision integrator (FP32 LU, one FP64
     refinement sweep per solve, per-cell FP64 fallback counted in the int32
-    [ncells] ``fallbacks``).
+    [ncells] ``fallbacks``).  ``lds_entries`` (persistent kernel, 0/20/24)
+    keeps the factored values of that many LU entries per cell in LDS.
     """
     import torch
     L = library()
@@ -269,7 +271,7 @@ def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, R
     RPAR = np.zeros(20) if RPAR is None else np.ascontiguousarray(RPAR, np.float64)
     atol = np.ascontiguousarray(np.atleast_1d(atol), np.float64)
     rtol = np.ascontiguousarray(np.atleast_1d(rtol), np.float64)
-    l = _Launch(variant, nblocks, refill_min, lanes, lu_fp32)
+    l = _Launch(variant, nblocks, refill_min, lanes, lu_fp32, lds_entries)
     nbytes = L.kpp_workspace_bytes(mech.encode(), int(IPAR[3]), ctypes.byref(l), n)
     if workspace is None:
         workspace = _default_ws.setdefault(str(dev), Workspace(dev))
@@ -328,7 +330,8 @@ def rates(mech, temp, cfactor, sun, time, out, stream=None, C=None):
 def autotune(mech: str, C, C0, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, RPAR=None,
              sun=None, sun_mode=SUN_FIXED, hstart=None, order=None, stream=None):
     """Fastest persistent-kernel launch shape on this batch: (variant, nblocks,
-    refill_min, lanes), milliseconds, candidates tried.  C is reset from C0."""
+    refill_min, lanes, lds_entries), milliseconds, candidates tried.  C is
+    reset from C0."""
     import torch
     L = library()
     _bind_extras(L)
@@ -363,7 +366,8 @@ def autotune(mech: str, C, C0, temp, cfactor, *, tstart, tend, atol, rtol, IPAR,
                        ctypes.byref(tried))
     if r != 0:
         raise KppError(f"kpp_autotune failed with {r}")
-    return ((best.variant, best.nblocks, best.refill_min, best.lanes), float(ms.value),
+    return ((best.variant, best.nblocks, best.refill_min, best.lanes, best.lds_entries),
+            float(ms.value),
             int(tried.value))
 
 

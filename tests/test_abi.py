@@ -96,3 +96,21 @@ def test_register_variant_geometry():
     # and not with the FP32 factorisation
     assert solver.workspace_bytes("small_strato", solver.ROS3, 4096, variant=solver.REGISTER,
                                   lu_fp32=1) == 0
+
+
+def test_lds_mirror_geometry():
+    # saprc99 and saprcnov are built with 20- and 24-entry LDS mirrors for the
+    # persistent FP64 kernel; the tiles are unchanged, so is the workspace
+    for mech in ("saprc99", "saprcnov"):
+        w0 = solver.workspace_bytes(mech, solver.RODAS3, 4096, variant=solver.PERSISTENT)
+        for lds in (20, 24):
+            assert solver.workspace_bytes(mech, solver.RODAS3, 4096, variant=solver.PERSISTENT,
+                                          lds_entries=lds) == w0 > 0
+    # refused: other sizes, other kernels, the FP32 factorisation, a mechanism
+    # built without a mirror
+    for mech, kw in (("saprc99", {"variant": solver.PERSISTENT, "lds_entries": 32}),
+                     ("saprc99", {"variant": solver.PERSISTENT, "lds_entries": -1}),
+                     ("saprc99", {"variant": solver.PLAIN, "lds_entries": 24}),
+                     ("saprc99", {"variant": solver.PERSISTENT, "lds_entries": 24, "lu_fp32": 1}),
+                     ("small_strato", {"variant": solver.PERSISTENT, "lds_entries": 20})):
+        assert solver.workspace_bytes(mech, solver.RODAS3, 4096, **kw) == 0, (mech, kw)

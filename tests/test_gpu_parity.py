@@ -370,3 +370,40 @@ def test_register_kernel_refused_for_large_mechanisms(gpu):
     b = workloads.saprc99_uneven(ncells=64)
     with pytest.raises(solver.KppError):
         _gpu_run(b, gpu, variant=solver.REGISTER)
+
+
+# ------------------------------------------------------------ LDS mirror of LU entries
+@pytest.mark.parametrize("lds", [20, 24])
+@pytest.mark.parametrize("wl,method", [("saprc99", solver.RODAS3), ("saprc99", solver.ROS2),
+                                       ("saprc99", solver.RODAS4), ("saprcnov", solver.RODAS3)])
+def test_lds_mirror_bit_identical_to_plain(gpu, lds, wl, method):
+    """Persistent kernel with the factored values of the most re-read LU
+    entries kept in LDS (KppDecompL/KppSolveL): the same operations in the
+    same order, so every cell's state, counters, Texit and Hexit are bit for
+    bit the plain kernel's, for 2-, 4- and 6-stage methods."""
+    if wl == "saprc99":
+        b = workloads.saprc99_uneven(ncells=3000, method=method)  # not a multiple of 64
+    else:
+        b = workloads.saprcnov_uneven(ncells=1500, method=method)
+    g0 = _gpu_run(b, gpu, variant=solver.PLAIN)
+    for refill in (1, 16):
+        g1 = _gpu_run(b, gpu, variant=solver.PERSISTENT, nblocks=11, refill_min=refill,
+                      lds_entries=lds)
+        assert g1["code"] == 0 and (g1["ierr"] == g0["ierr"]).all()
+        assert np.array_equal(g1["C"], g0["C"])
+        assert (g1["istat"] == g0["istat"]).all()
+        assert np.array_equal(g1["rstat"], g0["rstat"])
+
+
+def test_lds_mirror_refused(gpu):
+    """A mirror the mechanism is not built with, a size other than 0/20/24,
+    or a mirror outside the FP64 persistent kernel is an argument error."""
+    big = workloads.saprc99_uneven(ncells=64)
+    small = workloads.small_strato_tod(ncells=64)
+    for b, kw in ((small, {"variant": solver.PERSISTENT, "lds_entries": 24}),
+                  (big, {"variant": solver.PERSISTENT, "lds_entries": 32}),
+                  (big, {"variant": solver.PLAIN, "lds_entries": 24}),
+                  (big, {"variant": solver.PERSISTENT, "lds_entries": 24, "lu_fp32": 1})):
+        with pytest.raises(solver.KppError):
+            _gpu_run(b, gpu, **kw)
+

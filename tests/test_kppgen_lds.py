@@ -1,0 +1,116 @@
+"""The LDS-mirror variants of the LU routines (KppDecompL<NL>/KppSolveL<NL>),
+compiled for the host from the generated HIP model header.
+
+They are KppDecomp/KppSolve with the factored values of the NL most re-read
+LU entries kept in a second array (the lane's LDS slots on the GPU); the bar
+is bitwise equality with KppDecomp/KppSolve for every mirror size up to
+LDS_MAX, on random diagonally dominant matrices of the saprc99 and saprcnov
+patterns: the triangular solves read every LU entry, so any entry stored
+to or read from the wrong place shows in X.
+"""
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import ref
+
+GEN = Path(__file__).resolve().parents[1] / "kpp_amd" / "_gen"
+
+SHIM = """#pragma once
+#include <cmath>
+#define __device__
+#define __forceinline__ inline
+#define __noinline__
+#define __builtin_amdgcn_sched_barrier(x) ((void)0)
+using std::exp; using std::pow; using std::log; using std::sqrt;
+"""
+
+PROG = """#include "%(mech)s_hip.h"
+struct HA {
+  using value_type = double;
+  double* b;
+  double& operator[](int i) const { return b[i]; }
+};
+using M = kppm_%(mech)s;
+template <int NL>
+static void run(double* G, double* Gs, double* X) {
+  M::KppDecompL<NL>(HA{G}, HA{Gs});
+  M::KppSolveL<NL>(HA{G}, HA{X}, HA{Gs});
+}
+extern "C" int lds_max() { return M::LDS_MAX; }
+extern "C" void ref_lu(double* G, double* X) {
+  M::KppDecomp(HA{G});
+  M::KppSolve(HA{G}, HA{X});
+}
+extern "C" int split_lu(int nl, double* G, double* Gs, double* X) {
+  switch (nl) {
+    case 0: run<0>(G, Gs, X); return 0;
+    case 1: run<1>(G, Gs, X); return 0;
+    case 7: run<7>(G, Gs, X); return 0;
+    case 20: run<20>(G, Gs, X); return 0;
+    case 24: run<24>(G, Gs, X); return 0;
+  }
+  return -1;
+}
+"""
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@pytest.fixture(scope="module", params=["saprc99", "saprcnov"])
+def host_lu(request, tmp_path_factory):
+    mech = request.param
+    hdr = GEN / f"{mech}_hip.h"
+    if not hdr.exists():
+        pytest.skip(f"{hdr} not generated (run the build first)")
+    d = tmp_path_factory.mktemp(f"lds_{mech}")
+    (d / "hip").mkdir()
+    (d / "hip" / "hip_runtime.h").write_text(SHIM)
+    (d / "prog.cpp").write_text(PROG % {"mech": mech})
+    so = d / "liblds.so"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-shared", "-fPIC",
+                    "-I", str(d), "-I", str(GEN), str(d / "prog.cpp"), "-o", str(so)],
+                   check=True)
+    L = ctypes.CDLL(str(so))
+    for fn in (L.ref_lu, L.split_lu):
+        fn.restype = ctypes.c_int if fn is L.split_lu else None
+    return mech, L
+
+
+def _matrix(mech, rng):
+    s = ref.sizes(mech)
+    nv = s["NVAR"]
+    icol = np.zeros(s["LU_NONZERO"], np.int32)
+    crow = np.zeros(nv + 1, np.int32)
+    diag = np.zeros(nv + 1, np.int32)
+    ref.lib(mech).ros_ref_lu_pattern(_p(icol), _p(crow), _p(diag))
+    G = rng.uniform(-1.0, 1.0, s["LU_NONZERO"])
+    for i in range(nv):
+        G[diag[i]] = 2.0 + np.abs(G[crow[i]:crow[i + 1]]).sum()
+    return G, rng.uniform(-1.0, 1.0, nv)
+
+
+def test_split_lu_bitwise(host_lu):
+    mech, L = host_lu
+    assert L.lds_max() == 24
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        G0, X0 = _matrix(mech, rng)
+        Gr, Xr = G0.copy(), X0.copy()
+        L.ref_lu(_p(Gr), _p(Xr))
+        for nl in (0, 1, 7, 20, 24):
+            G, X = G0.copy(), X0.copy()
+            Gs = np.full(64, np.nan)
+            assert L.split_lu(nl, _p(G), _p(Gs), _p(X)) == 0
+            assert np.array_equal(X, Xr), nl
+            # entries not mirrored hold their factored values in G; the
+            # mirrored ones keep the unfactored value there
+            moved = np.nonzero(G != Gr)[0]
+            assert len(moved) <= nl
+            assert set(Gr[moved]) <= set(Gs[:nl])
+            assert np.isnan(Gs[nl:]).all()
