@@ -1685,24 +1685,96 @@ void emit_hip(Model& md, const std::string& path) {
   o << "}\n";
   o << "template <class TV, class TF, class TR, class TO>\nstatic __device__ __forceinline__ void "
        "Fun(TV V, TF F, TR RCT, TO Vdot) {\n";
-  std::set<int> used(md.a_used.begin(), md.a_used.end());
-  for (size_t q = 0; q < md.a_used.size(); q++)
-    o << "  const double A" << md.a_used[q] << " = " << md.a_expr[q] << ";\n";
-  for (int i = 0; i < S.nvar; i++) {
-    std::string e = md.vdot_expr[i];
-    // A[j] -> Aj (locals)
-    std::string r;
-    for (size_t p = 0; p < e.size(); p++) {
-      if (e[p] == 'A' && p + 1 < e.size() && e[p + 1] == '[' &&
-          (p == 0 || !(isalnum((unsigned char)e[p - 1]) || e[p - 1] == '_'))) {
-        size_t q = e.find(']', p);
-        r += "A" + e.substr(p + 2, q - p - 2);
-        p = q;
-      } else {
-        r += e[p];
+  {
+    std::vector<std::string> chain(S.nvar);
+    for (int i = 0; i < S.nvar; i++) {
+      const std::string& e = md.vdot_expr[i];
+      // A[j] -> Aj (locals)
+      std::string r;
+      for (size_t p = 0; p < e.size(); p++) {
+        if (e[p] == 'A' && p + 1 < e.size() && e[p + 1] == '[' &&
+            (p == 0 || !(isalnum((unsigned char)e[p - 1]) || e[p - 1] == '_'))) {
+          size_t q = e.find(']', p);
+          r += "A" + e.substr(p + 2, q - p - 2);
+          p = q;
+        } else {
+          r += e[p];
+        }
       }
+      chain[i] = r;
     }
-    o << "  Vdot[" << i << "] = " << r << ";\n";
+    // Reaction-major order.  Each Vdot[i] is a left-associated chain of terms
+    // [-][c*]Aj in increasing j; evaluating the chains term by term right
+    // after each Aj (a running value per species, stored when its last term
+    // is in) performs the same operations on the same operands in the same
+    // order, so the results are bitwise those of the chains, while only the
+    // open running values, not all NREACT products, are live at once.
+    struct Term { char op; std::string text; int j; };
+    auto parse = [](const std::string& r, std::vector<Term>& out) {
+      size_t p = 0;
+      while (p < r.size()) {
+        char op = 0;
+        if (r[p] == '+' || r[p] == '-') op = r[p++];
+        const size_t b = p;
+        if (p < r.size() && (isdigit((unsigned char)r[p]) || r[p] == '.')) {
+          while (p < r.size() && (isdigit((unsigned char)r[p]) || r[p] == '.')) p++;
+          if (p < r.size() && (r[p] == 'e' || r[p] == 'E')) {
+            p++;
+            if (p < r.size() && (r[p] == '+' || r[p] == '-')) p++;
+            while (p < r.size() && isdigit((unsigned char)r[p])) p++;
+          }
+          if (p >= r.size() || r[p] != '*') return false;
+          p++;
+        }
+        if (p >= r.size() || r[p] != 'A') return false;
+        p++;
+        const size_t d = p;
+        while (p < r.size() && isdigit((unsigned char)r[p])) p++;
+        if (p == d) return false;
+        Term t{op, r.substr(b, p - b), atoi(r.substr(d, p - d).c_str())};
+        if (out.empty() && op == '-') t.text = "-" + t.text;  // leading unary minus
+        if (out.empty()) t.op = 0;
+        else if (op == 0) return false;
+        if (!out.empty() && t.j <= out.back().j) return false;
+        out.push_back(t);
+      }
+      return true;
+    };
+    std::vector<std::vector<Term>> terms(S.nvar);
+    bool ok = true;
+    for (int i = 0; i < S.nvar && ok; i++)
+      if (chain[i].find('A') != std::string::npos) ok = parse(chain[i], terms[i]);
+    if (ok) {
+      // at[j]: (species, term index) pairs that use Aj
+      std::map<int, std::vector<std::pair<int, int>>> at;
+      for (int i = 0; i < S.nvar; i++)
+        for (int t = 0; t < (int)terms[i].size(); t++) at[terms[i][t].j].push_back({i, t});
+      for (int i = 0; i < S.nvar; i++)
+        if (terms[i].empty()) o << "  Vdot[" << i << "] = " << chain[i] << ";\n";
+      for (int i = 0; i < S.nvar; i++)
+        if (terms[i].size() > 1) o << "  double f" << i << ";\n";
+      for (size_t q = 0; q < md.a_used.size(); q++) {
+        const int j = md.a_used[q];
+        o << "  { const double A" << j << " = " << md.a_expr[q] << ";";
+        for (auto& it : at[j]) {
+          const int i = it.first;
+          const Term& t = terms[i][it.second];
+          const std::string v = "f" + std::to_string(i);
+          if (t.op == 0)
+            o << (terms[i].size() == 1 ? " Vdot[" + std::to_string(i) + "] = " + t.text + ";"
+                                       : " " + v + " = " + t.text + ";");
+          else
+            o << " " << v << " = " << v << t.op << t.text << ";";
+          if (terms[i].size() > 1 && it.second + 1 == (int)terms[i].size())
+            o << " Vdot[" << i << "] = " << v << ";";
+        }
+        o << " }\n";
+      }
+    } else {
+      for (size_t q = 0; q < md.a_used.size(); q++)
+        o << "  const double A" << md.a_used[q] << " = " << md.a_expr[q] << ";\n";
+      for (int i = 0; i < S.nvar; i++) o << "  Vdot[" << i << "] = " << chain[i] << ";\n";
+    }
   }
   o << "}\n";
   // Jac_SP fused with the Ghimj build of ros_PrepareMatrix

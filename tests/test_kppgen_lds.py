@@ -1,5 +1,6 @@
-"""The LDS-mirror variants of the LU routines (KppDecompL<NL>/KppSolveL<NL>),
-compiled for the host from the generated HIP model header.
+"""Routines of the generated HIP model header compiled for the host: the
+LDS-mirror variants of the LU routines (KppDecompL<NL>/KppSolveL<NL>) and the
+reaction-major Fun.
 
 They are KppDecomp/KppSolve with the factored values of the NL most re-read
 LU entries kept in a second array (the lane's LDS slots on the GPU); the bar
@@ -7,6 +8,11 @@ is bitwise equality with KppDecomp/KppSolve for every mirror size up to
 LDS_MAX, on random diagonally dominant matrices of the saprc99 and saprcnov
 patterns: the triangular solves read every LU entry, so any entry stored
 to or read from the wrong place shows in X.
+
+Fun is emitted reaction by reaction (each product term added to the running
+sums of the species it feeds right after it is formed); the bar is bitwise
+equality with the oracle's Fun, which evaluates each species' sum as one
+left-associated chain in reaction order (KPP's form).
 """
 import ctypes
 import subprocess
@@ -41,6 +47,9 @@ static void run(double* G, double* Gs, double* X) {
   M::KppSolveL<NL>(HA{G}, HA{X}, HA{Gs});
 }
 extern "C" int lds_max() { return M::LDS_MAX; }
+extern "C" void hip_fun(double* V, double* F, double* R, double* out) {
+  M::Fun(HA{V}, HA{F}, HA{R}, HA{out});
+}
 extern "C" void ref_lu(double* G, double* X) {
   M::KppDecomp(HA{G});
   M::KppSolve(HA{G}, HA{X});
@@ -114,3 +123,21 @@ def test_split_lu_bitwise(host_lu):
             assert len(moved) <= nl
             assert set(Gr[moved]) <= set(Gs[:nl])
             assert np.isnan(Gs[nl:]).all()
+
+
+def test_reaction_major_fun_bitwise(host_lu):
+    mech, L = host_lu
+    s = ref.sizes(mech)
+    nv, nf, nr = s["NVAR"], s["NFIX"], s["NREACT"]
+    R = ref.lib(mech)
+    rng = np.random.default_rng(13)
+    for _ in range(5):
+        # magnitudes spread over many decades, as in a chemistry state
+        V = 10.0 ** rng.uniform(-8, 12, nv)
+        F = 10.0 ** rng.uniform(8, 19, max(nf, 1))
+        K = 10.0 ** rng.uniform(-20, 2, nr)
+        a, b = np.zeros(nv), np.zeros(nv)
+        L.hip_fun(_p(V), _p(F), _p(K), _p(a))
+        R.ros_ref_fun(_p(V), _p(F), _p(K), _p(b))
+        assert np.array_equal(a, b)
+
