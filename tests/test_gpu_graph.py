@@ -25,7 +25,10 @@ def _batch(gpu, n, seed):
     return b, b.to_device(gpu)
 
 
-def test_graph_replays_match_eager(gpu):
+@pytest.mark.parametrize("lds", [0, 20])
+def test_graph_replays_match_eager(gpu, lds):
+    """Replays of the recorded tuned step (scheduler + persistent kernel, with
+    and without the LDS mirror of the LU) match the eager step bit for bit."""
     import torch
     b, d = _batch(gpu, 8192, 51)
     model = learned.StepModel.load(WEIGHTS, gpu)
@@ -47,7 +50,7 @@ def test_graph_replays_match_eager(gpu):
             b.mech, C, d["temp"], d["cfactor"], tstart=d["tstart"], tend=d["tend"], atol=b.atol,
             rtol=b.rtol, IPAR=b.IPAR, RPAR=b.RPAR, sun=d["sun"], sun_mode=b.sun_mode,
             hstart=sched[0], order=sched[2], ierr=ierr, istat=istat, rstat=rstat,
-            variant=solver.PERSISTENT, workspace=ws)
+            variant=solver.PERSISTENT, lds_entries=lds, workspace=ws)
         assert code == 0
 
     C.copy_(C0)
