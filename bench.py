@@ -13,11 +13,15 @@ and reported beside it, and the CPU oracle (this project's C restatement of
 KPP's int/rosenbrock.c, OpenMP over cells) is timed on a bounded sample of
 the same cells on the host's cores.
 
+Also timed on the same cells: the persistent kernel with each cell's
+starting step estimated from f(t0, y0) (IPAR[4] = 1, the non-learned
+alternative to the scheduler), and, for mechanisms built with one, the
+cooperative kernel (one wavefront per cell, the LU in LDS).
+
 Accuracy (untimed): on an evenly spaced sample of cells, the final states of
-the plain and tuned runs are compared with an FP64 integration of the same
-cells at RTOL 1e-10 (ATOL 1e-3) by the plain kernel; max / median / 99th
-percentile relative errors are reported for species above ATOL and above
-ATOL/RTOL.
+every run are compared with the CPU oracle's FP64 integration of the same
+cells at RTOL 1e-10 (ATOL 1e-3); max / median / 99th percentile relative
+errors are reported for species above ATOL and above ATOL/RTOL.
 
 Roofline: the solver is bound by HBM bandwidth.  Algorithmic bytes per step
 attempt (one unit) follow the streamed working set of each attempt:
@@ -88,14 +92,56 @@ def algorithmic_work(mech, method, istat, mixed=False):
             first + retry, {"bytes_first_attempt": b_first, "bytes_retry": b_retry})
 
 
+def host_cpu():
+    """CPU model, physical cores of the host and of this process's affinity set."""
+    model, cores = None, set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            txt = f.read()
+        for blk in txt.strip().split("\n\n"):
+            kv = dict(l.split(":", 1) for l in blk.splitlines() if ":" in l)
+            kv = {k.strip(): v.strip() for k, v in kv.items()}
+            model = model or kv.get("model name")
+            cores.add((kv.get("physical id"), kv.get("core id")))
+    except OSError:
+        pass
+    aff = sorted(os.sched_getaffinity(0))
+    aff_cores = set()
+    for c in aff:
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/core_id") as f:
+                core = f.read().strip()
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/physical_package_id") as f:
+                pkg = f.read().strip()
+            aff_cores.add((pkg, core))
+        except OSError:
+            aff_cores.add(("?", c))
+    return {"model": model, "host_physical_cores": len(cores) or None,
+            "affinity_logical_cpus": len(aff), "affinity_physical_cores": len(aff_cores)}
+
+
 def cpu_baseline(wl_fn, ncells_full, seed, target_s=20.0):
-    """Time the oracle on a bounded sample of the same workload."""
+    """Time the oracle on a bounded sample of the same workload: built here
+    with gcc -O2 -march=native, OpenMP threads = the CPUs this process may
+    use, bound close to cores."""
+    import tempfile
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import ref
-    threads = len(os.sched_getaffinity(0))
+    cpu = host_cpu()
+    threads = cpu["affinity_logical_cpus"]
+    mech = wl_fn(ncells=1, seed=seed).mech
+    try:
+        path = ref.build_native(mech, tempfile.mkdtemp(prefix="kpp_oracle_"))
+        flags = "gcc -O2 -march=native -fopenmp"
+    except Exception as e:  # no compiler: the prebuilt portable timing build
+        log(f"[bench] native oracle build failed ({e}); using the x86-64-v3 build")
+        path, flags = None, "gcc -O2 -march=x86-64-v3 -fopenmp"
+    run = lambda bb: ref.integrate(bb, nthreads=threads, fast=path is None, path=path)
     ncal = max(1024, 128 * threads)
     cal = wl_fn(ncells=ncal, seed=seed)
     t = time.time()
-    ref.integrate(cal, nthreads=threads, fast=True)
+    run(cal)
     rate = ncal / max(time.time() - t, 1e-3)
     n = int(min(ncells_full, max(1024, rate * target_s)))
     for _ in range(2):
@@ -103,7 +149,7 @@ def cpu_baseline(wl_fn, ncells_full, seed, target_s=20.0):
         cells = np.linspace(0, ncells_full - 1, n).astype(np.int64)
         b = wl_fn(seed=seed, cells=cells)
         t = time.time()
-        out = ref.integrate(b, nthreads=threads, fast=True)
+        out = run(b)
         dt = time.time() - t
         # a calibration that overestimated the time per cell: size the sample again
         if dt >= 0.4 * target_s or n >= ncells_full:
@@ -111,8 +157,11 @@ def cpu_baseline(wl_fn, ncells_full, seed, target_s=20.0):
         n = int(min(ncells_full, n * target_s / max(dt, 1e-3)))
     return {"value": n / dt, "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": f"{n} evenly spaced cells of the same {ncells_full}-cell batch, "
-                      f"oracle ros_ref (gcc -O2 -march=x86-64-v3, OpenMP dynamic schedule), "
-                      f"{dt:.1f} s", "ierr_ok": bool((out["ierr"] == 1).all())}
+                      f"oracle ros_ref ({flags}, OpenMP dynamic schedule, "
+                      f"OMP_PROC_BIND={os.environ['OMP_PROC_BIND']} "
+                      f"OMP_PLACES={os.environ['OMP_PLACES']}), {dt:.1f} s",
+            "cpu": cpu, "threads": threads, "compiler_flags": flags,
+            "ierr_ok": bool((out["ierr"] == 1).all())}
 
 
 def copy_probe(nbytes=1 << 30, reps=3):
@@ -135,6 +184,8 @@ def copy_probe(nbytes=1 << 30, reps=3):
 
 
 def measured_traffic(workload, ncells, kernel):
+    """HBM bytes per launch of this kernel on this workload from the committed
+    rocprofv3 counter passes (not measured in this run), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
@@ -170,6 +221,10 @@ def main():
     ap.add_argument("--refill-min", type=int, default=0)
     ap.add_argument("--weights", default=WEIGHTS, help="trained step-scheduler network")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-coop", action="store_true",
+                    help="skip the cooperative kernel (mechanisms built with one)")
+    ap.add_argument("--no-heuristic", action="store_true",
+                    help="skip the run with estimated starting steps (IPAR[4] = 1)")
     ap.add_argument("--accuracy-cells", type=int, default=4096)
     ap.add_argument("--eager-tuned", action="store_true",
                     help="run the tuned path as eager calls instead of one graph (same kernels; "
@@ -226,12 +281,16 @@ def main():
     # the graph path has made it, else --lds-entries (for counter runs)
     tuned_launch = {"lds_entries": args.lds_entries} if args.lds_entries else {}
 
-    def step(variant, learned_path, lu_fp32=0):
+    def step(variant, learned_path, lu_fp32=0, hstart_auto=False):
         C.copy_(C0)
         kw = {"nblocks": args.nblocks if variant == solver.PERSISTENT else 0,
               "refill_min": args.refill_min}
-        if learned_path and variant == solver.PERSISTENT:
+        if (learned_path or hstart_auto) and variant == solver.PERSISTENT:
             kw.update(tuned_launch)
+        ipar = b.IPAR
+        if hstart_auto:
+            ipar = b.IPAR.copy()
+            ipar[4] = 1
         ev0, evk, ev1 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         ev0.record()
         hs = order = None
@@ -241,21 +300,21 @@ def main():
         evk.record()
         code, _, _, _ = solver.rosenbrock(
             b.mech, C, d["temp"], d["cfactor"], tstart=d["tstart"], tend=d["tend"],
-            atol=b.atol, rtol=b.rtol, IPAR=b.IPAR, RPAR=b.RPAR, sun=d["sun"],
+            atol=b.atol, rtol=b.rtol, IPAR=ipar, RPAR=b.RPAR, sun=d["sun"],
             sun_mode=b.sun_mode, hstart=hs, order=order, ierr=ierr, istat=istat, rstat=rstat,
             variant=variant, lu_fp32=lu_fp32, fallbacks=fallbacks, workspace=ws, **kw)
         ev1.record()
         assert code == 0, code
         return ev0, evk, ev1
 
-    def timed(variant, learned_path, steps, warmup, name, lu_fp32=0):
+    def timed(variant, learned_path, steps, warmup, name, lu_fp32=0, hstart_auto=False):
         for _ in range(warmup):
-            step(variant, learned_path, lu_fp32)
+            step(variant, learned_path, lu_fp32, hstart_auto)
         torch.cuda.synchronize()
         evs = []
         t = time.perf_counter()
         for k in range(steps):
-            evs.append(step(variant, learned_path, lu_fp32))
+            evs.append(step(variant, learned_path, lu_fp32, hstart_auto))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
         dev_t = [a.elapsed_time(e) / 1e3 for a, _, e in evs]
@@ -345,9 +404,21 @@ def main():
             runs["tuned"]["kernel_s"] = runs["tuned"]["device_s"] - eager["sched_s"]
     runs["persistent"] = timed(solver.PERSISTENT, False, max(1, args.steps if model is None else 1),
                                args.warmup, "persistent, default start")
+    if not args.no_heuristic:
+        # the non-learned alternative: each cell's starting step estimated from
+        # f(t0, y0) (Hairer-Norsett-Wanner, IPAR[4] = 1), same launch shape as
+        # the tuned path, cells in index order
+        runs["heuristic"] = timed(solver.PERSISTENT, False, max(1, args.steps if model is None else 1),
+                                  args.warmup, "persistent, estimated start (IPAR[4]=1)",
+                                  hstart_auto=True)
     if args.lu_fp32:
         runs["mixed"] = timed(solver.PERSISTENT, False, max(1, args.steps), args.warmup,
                               "persistent, FP32 LU + FP64 refinement", lu_fp32=1)
+    # large mechanisms: one wavefront per cell, the LU in LDS
+    coop_ok = solver.workspace_bytes(b.mech, b.method, n, variant=solver.COOP) > 0
+    if coop_ok and model is None and not args.no_coop:
+        runs["coop"] = timed(solver.COOP, False, max(1, args.steps), args.warmup,
+                             "cooperative kernel (one wavefront per cell, LU in LDS)")
     # small mechanisms: the kernel that keeps each cell's working set in VGPRs
     reg_ok = solver.workspace_bytes(b.mech, b.method, n, variant=solver.REGISTER) > 0
     if reg_ok and model is None and not args.lu_fp32:
@@ -356,37 +427,42 @@ def main():
     if not args.no_plain:
         runs["plain"] = timed(solver.PLAIN, False, max(1, args.steps), max(0, args.warmup), "plain")
     top = runs["tuned"] if model is not None else (
-        runs["mixed"] if args.lu_fp32 else runs.get("register", runs["persistent"]))
-    top_register = model is None and not args.lu_fp32 and "register" in runs
+        runs["mixed"] if args.lu_fp32 else
+        runs.get("coop", runs.get("register", runs["persistent"])))
+    top_register = model is None and not args.lu_fp32 and "register" in runs and "coop" not in runs
+    top_coop = model is None and not args.lu_fp32 and "coop" in runs
 
     # roofline of the persistent kernel on the tuned run's attempts
     top_mixed = (model is None and args.lu_fp32)
     top_lds = tuned_launch.get("lds_entries", 0) if model is not None else 0
     top_kernel = (f"ros_persistent_kernel<kppm_{b.mech},{solver.METHOD_STAGES[b.method]}"
                   f"{',MIX' if top_mixed else ''}{',LDS' + str(top_lds) if top_lds else ''}>")
+    if top_coop:
+        top_kernel = f"ros_coop_kernel<kppm_{b.mech},{solver.METHOD_STAGES[b.method]}>"
     nbytes, nflops, attempts, bterms = algorithmic_work(mech, b.method, top["istat"], top_mixed)
     kern_t = top["kernel_s"]
 
     accuracy = None
     if args.accuracy_cells > 0:
+        # reference: the CPU oracle (the restatement of KPP's C Rosenbrock) in
+        # FP64 at RTOL 1e-10, ATOL 1e-3 on an evenly spaced sample of the cells
         import dataclasses
+
+        from oracle import ref
         cells = np.linspace(0, n - 1, min(n, args.accuracy_cells)).astype(np.int64)
         tb = wl_fn(seed=args.seed, cells=cells)
         tb = dataclasses.replace(tb, atol=np.full_like(tb.atol, 1e-3),
                                  rtol=np.full_like(tb.rtol, 1e-10))
-        td = tb.to_device(dev)
         t = time.time()
-        code, terr, _, _ = solver.rosenbrock(
-            tb.mech, td["C"], td["temp"], td["cfactor"], tstart=td["tstart"], tend=td["tend"],
-            atol=tb.atol, rtol=tb.rtol, IPAR=tb.IPAR, RPAR=tb.RPAR, sun=td["sun"],
-            sun_mode=tb.sun_mode, variant=solver.PERSISTENT)
-        torch.cuda.synchronize()
-        ref_C = td["C"].cpu().numpy()
-        log(f"[bench] accuracy reference (RTOL 1e-10, ATOL 1e-3, {len(cells)} cells) in "
-            f"{time.time() - t:.1f}s, all IERR=1: {bool((terr == 1).all().item())}")
+        o = ref.integrate(tb, nthreads=len(os.sched_getaffinity(0)), fast=True)
+        ref_C = o["C"]
+        log(f"[bench] accuracy reference: CPU oracle, RTOL 1e-10, ATOL 1e-3, {len(cells)} cells "
+            f"in {time.time() - t:.1f}s, all IERR=1: {bool((o['ierr'] == 1).all())}")
         at, rt = float(b.atol[0]), float(b.rtol[0])
-        accuracy = {"reference": f"FP64 RTOL 1e-10 ATOL 1e-3 integration of {len(cells)} "
-                                 f"evenly spaced cells", "sample_cells": int(len(cells))}
+        accuracy = {"reference": f"CPU oracle (ros_ref, FP64) at RTOL 1e-10 ATOL 1e-3 on "
+                                 f"{len(cells)} evenly spaced cells of the batch",
+                    "reference_all_ierr_1": bool((o["ierr"] == 1).all()),
+                    "sample_cells": int(len(cells))}
         for k, r in runs.items():
             accuracy[k] = rel_errors(r["C"][:, cells], ref_C, mech.nvar, at, rt)
         if "mixed" in runs:
@@ -463,12 +539,19 @@ def main():
                             if model is not None else
                             ("persistent work-stealing kernel, FP32 LU + FP64 refinement"
                              if args.lu_fp32 else
-                             ("register kernel: one lane per cell, working set in VGPRs"
-                              if top_register else "persistent work-stealing kernel"))),
+                             ("cooperative kernel: one wavefront per cell, LU in LDS"
+                              if top_coop else
+                              ("register kernel: one lane per cell, working set in VGPRs"
+                               if top_register else "persistent work-stealing kernel")))),
                    "all_ierr_1": top["ok"], "attempts_per_cell": attempts / n},
         "roofline": roof or {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, n, top_kernel),
+                     "traffic_source": (os.path.relpath(TRAFFIC_FILE, HERE) +
+                                        ": rocprofv3 FETCH_SIZE/WRITE_SIZE passes of this kernel "
+                                        "on this workload, not measured in this run"
+                                        if measured_traffic(args.workload, n, top_kernel)
+                                        else None),
                      "kernel": top_kernel,
                      "kernel_ms": kern_t * 1e3, "algorithmic_bytes": nbytes,
                      "fp64_tflops": nflops / kern_t / 1e12,
@@ -479,12 +562,22 @@ def main():
         "plain": ({"value": n / plain["wall"], "kernel_ms": plain["kernel_s"] * 1e3,
                    "attempts_per_cell": float(plain["istat"][2].mean()),
                    "all_ierr_1": plain["ok"]} if plain else None),
+        "coop": ({"value": n / runs["coop"]["wall"], "kernel_ms": runs["coop"]["kernel_s"] * 1e3,
+                  "attempts_per_cell": float(runs["coop"]["istat"][2].mean()),
+                  "speedup_vs_persistent": runs["persistent"]["wall"] / runs["coop"]["wall"],
+                  "all_ierr_1": runs["coop"]["ok"]} if "coop" in runs else None),
         "register": ({"value": n / runs["register"]["wall"],
                       "kernel_ms": runs["register"]["kernel_s"] * 1e3,
                       "attempts_per_cell": float(runs["register"]["istat"][2].mean()),
                       "all_ierr_1": runs["register"]["ok"]} if "register" in runs else None),
         "persistent_default_start": {"value": n / runs["persistent"]["wall"],
                                      "attempts_per_cell": float(runs["persistent"]["istat"][2].mean())},
+        "persistent_estimated_start": ({"value": n / runs["heuristic"]["wall"],
+                                        "attempts_per_cell":
+                                            float(runs["heuristic"]["istat"][2].mean()),
+                                        "launch": dict(tuned_launch) or None,
+                                        "all_ierr_1": runs["heuristic"]["ok"]}
+                                       if "heuristic" in runs else None),
         "mixed": ({"value": n / runs["mixed"]["wall"],
                    "kernel_ms": runs["mixed"]["kernel_s"] * 1e3,
                    "attempts_per_cell": float(runs["mixed"]["istat"][2].mean()),

@@ -14,9 +14,19 @@
  *   IPAR[0] autonomous, IPAR[1] scalar tolerances (0 = vector), IPAR[2] max
  *   steps, IPAR[3] method (1 Ros2, 2 Ros3, 3 Ros4, 4 Rodas3, 5 Rodas4),
  *   IPAR[10..17] Nfun Njac Nstp Nacc Nrej Ndec Nsol Nsng on entry;
+ *   IPAR[4] (unused by the reference) = 1: where no starting step is given
+ *   (RPAR[2] or hstart[c] = 0), estimate one per cell (Hairer, Norsett &
+ *   Wanner, Solving ODEs I, II.4; two function evaluations, counted in Nfun)
+ *   instead of the reference's 1e-6 s;
  *   RPAR[0..6] Hmin Hmax Hstart FacMin FacMax FacRej FacSafe.
- * Per-cell results use Rosenbrock's codes (1 success, -1..-8 failures);
- * per-cell statistics come back in the IPAR[10..17] order, Texit/Hexit in
+ * Per-cell results use Rosenbrock's codes (1 success, -1..-8 failures) and
+ * two of this library's own, for cells whose reject loop would never end
+ * (int/rosenbrock.c:537-619 has no bound; a GPU wavefront must finish):
+ *   -9  the error estimate is NaN or the new step size is not finite (a NaN
+ *       or Inf in the cell's inputs, rates or Jacobian);
+ *  -10  more than 100000 rejected attempts in a row (step factors that do
+ *       not shrink the step, e.g. FacMin = FacRej = 1).
+ * Per-cell statistics come back in the IPAR[10..17] order, Texit/Hexit in
  * the RPAR[10..11] order.  Parameter errors (-1..-5) are returned by the
  * call itself, as Rosenbrock returns them before integrating.  API misuse
  * returns KPP_E_* (<= -100).  Nothing is printed.
@@ -38,6 +48,7 @@ extern "C" {
 #define KPP_E_HIP (-103)       /* HIP runtime error */
 #define KPP_E_WORKSPACE (-104) /* workspace too small */
 #define KPP_E_NODEV (-105)     /* no GPU device */
+#define KPP_E_WEIGHTS (-106)   /* scheduler weights file missing or not for this mechanism */
 
 #define KPP_SUN_FIXED 0 /* SUN given per cell and held fixed (Update_SUN bypassed) */
 #define KPP_SUN_TOD 1   /* SUN from Update_SUN(TIME) (util/UpdateSun.c:1-20) */
@@ -47,6 +58,11 @@ extern "C" {
 #define KPP_VARIANT_REGISTER 2   /* one lane per cell, working set in registers: small
                                     mechanisms only (<= 64 LU entries and reactions),
                                     else the call returns KPP_E_ARG */
+#define KPP_VARIANT_COOP 3       /* one wavefront per cell (its 64 lanes share the model
+                                    routines; the LU in LDS): large mechanisms; FP64, for
+                                    mechanisms built with cooperative programs, else
+                                    KPP_E_ARG.  Persistent grid: kpp_launch.nblocks
+                                    wavefronts (0 = all that fit), lanes 0 or 64 */
 
 /* ---- mechanisms compiled into the library ---- */
 int kpp_mech_count(void);
@@ -151,9 +167,31 @@ int kpp_rates_c_dev(const char* mech, int n, int ld, const double* temp, const d
 
 /* ---- host-buffer convenience API (allocates device memory itself) ---- */
 typedef struct kpp_ctx kpp_ctx;
-/* method: IPAR[3] code; flags: KPP_SUN_TOD to drive SUN by time of day */
+/* Context flags (or-ed with KPP_SUN_TOD).  None has a counterpart in KPP; the
+ * default (0) integrates every cell with the plain one-lane-per-cell kernel
+ * in KPP's FP64 arithmetic. */
+#define KPP_CTX_PERSISTENT 0x10  /* persistent work-stealing kernel, default launch shape */
+#define KPP_CTX_AUTOTUNE 0x20    /* persistent kernel with the launch shape kpp_autotune
+                                    picks on the context's first integration (it integrates
+                                    that batch once per candidate, then restores it) */
+#define KPP_CTX_LEARNED 0x40     /* learned step scheduler: starting step per cell where the
+                                    call gives none, and the launch order; weights from
+                                    <library dir>/<mech>_step_mlp.kppw (written by the
+                                    build) or kpp_ctx_load_scheduler; KPP_E_WEIGHTS if none */
+#define KPP_CTX_MIXED 0x80       /* FP32 LU with FP64 refinement (kpp_launch.lu_fp32 = 1);
+                                    not with KPP_CTX_AUTOTUNE */
+#define KPP_CTX_HSTART_AUTO 0x100 /* INTEGRATE_BATCH: estimated starting step (IPAR[4] = 1)
+                                     where STEPMIN is 0 */
+/* method: IPAR[3] code; flags: KPP_SUN_TOD to drive SUN by time of day, KPP_CTX_* */
 int kpp_ctx_create(kpp_ctx** ctx, const char* mech, int ncells, int method, int flags);
 void kpp_ctx_destroy(kpp_ctx* ctx);
+/* KPP_CTX_LEARNED: replace the scheduler's weights (a file kpp_sched_load reads) */
+int kpp_ctx_load_scheduler(kpp_ctx* ctx, const char* path);
+/* the launch shape the context integrates with (after autotuning, the chosen one) */
+int kpp_ctx_get_launch(kpp_ctx* ctx, kpp_launch* l);
+/* host [ncells]: the starting step each cell was given in the last integration
+ * (the caller's, the carried STEPMIN, the scheduler's; 0 where RPAR[2]'s rule applied) */
+int kpp_ctx_get_hstart(kpp_ctx* ctx, double* hstart);
 /* host SoA C[NSPEC][ncells]; per-cell temp, cfactor, sun (NULL in TOD mode) */
 int kpp_set_state(kpp_ctx* ctx, const double* C, const double* temp, const double* cfactor,
                   const double* sun);
@@ -250,6 +288,19 @@ size_t kpp_mlp_schedule_work_bytes(int n);
 int kpp_mlp_schedule(const kpp_mlp* net, const void* X, int n, float mu_y, float sd_y,
                      double hmin, double hmax, const float* centres, double* hstart, int* bucket,
                      int* order, void* work, size_t work_bytes, void* stream);
+
+/* The scheduler as one object for C hosts: network, input standardisation
+ * and step scale loaded from a weights file (the flat image kpp_amd/learned.py
+ * writes beside a trained .npz; the build exports <lib dir>/<mech>_step_mlp.kppw).
+ * kpp_sched_run writes, for the cells of a device batch, hstart[ncells] (the
+ * predicted first step times the trained scale, clamped to [hmin, hmax]) and
+ * order[ncells] (most expected steps first); either may be NULL.  Allocates
+ * its work buffers on first use. */
+typedef struct kpp_sched kpp_sched;
+int kpp_sched_load(const char* path, const char* mech, kpp_sched** out);
+void kpp_sched_destroy(kpp_sched* s);
+int kpp_sched_run(kpp_sched* s, const kpp_batch* b, double hmin, double hmax, double* hstart,
+                  int* order, void* stream);
 
 #ifdef __cplusplus
 }

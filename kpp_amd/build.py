@@ -44,6 +44,10 @@ MECHANISMS = {
     # inlined through #INLINE F90_RCONST), with the air density and its O2, N2,
     # H2O fractions and the solar zenith angle from examples/mcm/driver_mcm.f90
     # (cos(zenith) is the cell's SUN)
+    # this project's test mechanism (kpp_amd/mechs/linear6.kpp; an absolute
+    # path, so not under KPP_HOME): six independent linear species whose
+    # steps and singular iteration matrices have closed forms
+    "linear6": (str(PKG / "mechs" / "linear6.kpp"), []),
     "mcm": ("examples/mcm/mcm.kpp",
             ["--no-pipelined", "--part-ops", "3000", "--rates-f90", "--f90-module", "examples/mcm/constants_mcm.f90",
              "--env", "M = CFACTOR*1.0D6; O2 = 0.21*M; N2 = 0.78*M; H2O = 1.0E-2*M; "
@@ -70,7 +74,7 @@ HIP_FLAGS = [
 # cells fill one wavefront per SIMD at most), 2.03 s at 2 and 1.88 s at 1
 # (VGPRs plus AGPRs for spills).  small_strato: 2, so that its register
 # kernel (the whole working set in VGPRs, 71-86 doubles per lane) fits.
-WAVES_PER_EU = {"small_strato": 2, "saprc99": 2, "saprcnov": 2, "mcm": 1}
+WAVES_PER_EU = {"small_strato": 2, "saprc99": 2, "saprcnov": 2, "linear6": 2, "mcm": 1}
 
 
 def kpp_home() -> Path:
@@ -106,7 +110,8 @@ def generate(mechs=None) -> list[str]:
         args = []
         for k, a in enumerate(opts):
             args.append(str(home / a) if k > 0 and opts[k - 1] == "--f90-module" else a)
-        run([str(exe), "--kpp-home", str(home), "--root", root, "-o", str(GEN), *args, str(src)])
+        run([str(exe), "--kpp-home", str(home), "--root", root, "-o", str(GEN), "--coop", *args,
+             str(src)])
         (GEN / f"kpphip_{root}.hip").write_text(
             f'#include "{root}_hip.h"\n#include "mech_launch.h"\nKPPHIP_DEFINE_MECH({root})\n')
         # the mixed-precision integrator and the batched model functions in
@@ -126,10 +131,33 @@ def _mech_of(src: Path) -> str:
     return src.stem.replace("kpphip_", "", 1).replace("_mixed", "").replace("_comp", "")
 
 
+def _stamp(src: Path, cmd: list) -> str:
+    """Digest of a translation unit's inputs: the command, the source, the
+    library's headers and the mechanism's generated header."""
+    import hashlib
+    h = hashlib.sha256(" ".join(cmd).encode())
+    deps = [src, *sorted(CSRC.glob("*.h")), *sorted((REPO / "include").glob("*.h"))]
+    mh = GEN / (_mech_of(src) + "_hip.h")
+    if mh.exists():
+        deps.append(mh)
+    for d in deps:
+        h.update(d.read_bytes())
+    return h.hexdigest()
+
+
 def _hipcc_obj(src: Path, out: Path):
     waves = WAVES_PER_EU.get(_mech_of(src), 4)
-    run([HIPCC, *HIP_FLAGS, f"-DKPPHIP_WAVES_PER_EU={waves}", "-I", str(CSRC), "-I", str(GEN),
-         "-I", str(REPO / "include"), "-c", "-x", "hip", str(src), "-o", str(out)])
+    cmd = [HIPCC, *HIP_FLAGS, f"-DKPPHIP_WAVES_PER_EU={waves}", "-I", str(CSRC), "-I", str(GEN),
+           "-I", str(REPO / "include"), "-c", "-x", "hip", str(src), "-o", str(out)]
+    # KPPHIP_INCREMENTAL=1 (development): skip a translation unit whose
+    # inputs are unchanged since its object was built
+    stamp_file = out.with_suffix(".stamp")
+    stamp = _stamp(src, cmd) if os.environ.get("KPPHIP_INCREMENTAL") == "1" else None
+    if stamp and out.exists() and stamp_file.exists() and stamp_file.read_text() == stamp:
+        return out
+    run(cmd)
+    if stamp:
+        stamp_file.write_text(stamp)
     return out
 
 
@@ -139,7 +167,7 @@ def build_lib(mechs: list[str], jobs: int = 8) -> Path:
     srcs = ([GEN / f"kpphip_{m}.hip" for m in mechs] +
             [GEN / f"kpphip_{m}_mixed.hip" for m in mechs] +
             [GEN / f"kpphip_{m}_comp.hip" for m in mechs] +
-            [CSRC / "kpphip_api.cpp", CSRC / "mlp.hip"])
+            [CSRC / "kpphip_api.cpp", CSRC / "mlp.hip", CSRC / "sched.cpp"])
     # largest mechanism first, so the slowest translation units start at once
     def weight(src):
         h = GEN / (_mech_of(src) + "_hip.h")
@@ -181,10 +209,25 @@ def build_oracle(mechs: list[str]) -> list[Path]:
     return outs
 
 
+def export_weights(mechs: list[str]) -> list[Path]:
+    """The shipped scheduler weights (kpp_amd/weights/<mech>_step_mlp.npz) as
+    the flat images C hosts load (kpp_sched_load), next to the library."""
+    from . import learned
+    outs = []
+    for m in mechs:
+        src = PKG / "weights" / f"{m}_step_mlp.npz"
+        if src.exists():
+            out = LIB / f"{m}_step_mlp.kppw"
+            learned.export_flat(src, out)
+            outs.append(out)
+    return outs
+
+
 def build_all(with_oracle: bool = True) -> dict:
     mechs = generate()
     lib = build_lib(mechs)
-    info = {"lib": str(lib), "mechanisms": mechs, "driver": str(build_driver(lib))}
+    info = {"lib": str(lib), "mechanisms": mechs, "driver": str(build_driver(lib)),
+            "weights": [str(p) for p in export_weights(mechs)]}
     if with_oracle:
         info["oracle"] = [str(p) for p in build_oracle(mechs)]
     return info

@@ -1483,7 +1483,7 @@ struct PipeParams {
   int dist = 24, window = 48, chunk = 12;
 };
 PipeParams g_pipe;
-bool g_emit_pipelined = true;
+bool g_emit_pipelined = false;  // --pipelined; the integrator does not use them
 int g_lds_split = 0;   // LU entries the KppDecompL/KppSolveL variants may mirror in LDS (0: no variants)
 long g_part_ops = 0;  // split routines into out-of-line parts of this many operations (0: no)
 
@@ -1610,6 +1610,8 @@ void emit_pipelined(std::ostream& o, const Structure& S, const PipeParams& pp) {
     o << "}\n";
   }
 }
+
+void emit_coop(Model& md, std::ostream& o, const std::string& ns);
 
 // The HIP header: everything is a template over an element accessor, so the
 // same straight-line code runs on SoA global memory (p[i*stride]), LDS or
@@ -2102,6 +2104,395 @@ void emit_hip(Model& md, const std::string& path) {
     o << "};\n";
   }
   o << "}\n";
+  emit_coop(md, o, ns);
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative programs (--coop): one wavefront integrates one cell, its 64
+// lanes sharing the work of each model routine.  The routines become tables
+// that a generic device interpreter walks step by step:
+//  * products (A[j] of Fun, B[k] of Jac_SP): up to MAXF factors each, taken
+//    left to right as in the emitted code (RCT[j], V[i], F[i] or a constant);
+//    one product per lane at a time;
+//  * sums and eliminations as chains: every value the model routines compute
+//    as a left-associated chain (Vdot[i], JVS[n], an LU entry's updates and
+//    its division, a row of the triangular solves) is one chain, whose steps
+//    run consecutively on one lane with the running value in a register.
+//    Same operands, same order as the emitted straight-line code, so the
+//    results are bitwise the same.
+// Chains are list-scheduled on 64 lanes in order of their longest path to
+// the end; a step that reads another chain's result runs at least two steps
+// after that chain's last step (the interpreter issues the loads of step
+// t+1 before the store of step t).  Each table entry is one 64-bit
+// descriptor per (step, lane); an all-ones descriptor is an idle lane.
+// ---------------------------------------------------------------------------
+bool g_coop = false;
+
+namespace coop {
+
+constexpr uint64_t IDLE = ~0ull;
+// product factors: type in bits 30-31 (0 RCT, 1 V, 2 F, 3 constant), index below
+constexpr uint32_t F_RCT = 0u << 30, F_V = 1u << 30, F_F = 2u << 30, F_C = 3u << 30;
+constexpr uint32_t F_END = 0xFFFFFFFFu;
+
+struct Term {
+  double coef;
+  int src;
+};
+
+// "[-][c*]X[n]" terms of a printed chain over array X ("A" or "B"); false if
+// the text has another form
+bool parse_chain(const std::string& e, const char* X, std::vector<Term>& out) {
+  out.clear();
+  if (e == "0.0" || e == "0") return true;
+  size_t p = 0;
+  const std::string ref = std::string(X) + "[";
+  while (p < e.size()) {
+    double sign = 1.0;
+    if (e[p] == '+' || e[p] == '-') {
+      if (e[p] == '-') sign = -1.0;
+      p++;
+    } else if (!out.empty()) {
+      return false;
+    }
+    double c = 1.0;
+    if (e.compare(p, ref.size(), ref) != 0) {
+      char* end = nullptr;
+      c = strtod(e.c_str() + p, &end);
+      size_t q = end - e.c_str();
+      if (q == p || q >= e.size() || e[q] != '*') return false;
+      p = q + 1;
+    }
+    if (e.compare(p, ref.size(), ref) != 0) return false;
+    p += ref.size();
+    size_t q = e.find(']', p);
+    if (q == std::string::npos) return false;
+    out.push_back({sign * c, atoi(e.substr(p, q - p).c_str())});
+    p = q + 1;
+  }
+  return true;
+}
+
+// a printed product "f*f*..." of RCT[j], V[i], F[i] and constants
+bool parse_product(const std::string& e, std::vector<uint32_t>& fac, std::vector<double>& pool) {
+  fac.clear();
+  size_t p = 0;
+  while (p <= e.size()) {
+    size_t q = e.find('*', p);
+    std::string f = e.substr(p, q == std::string::npos ? std::string::npos : q - p);
+    auto idx = [&](const char* pre, uint32_t type) -> bool {
+      size_t n = strlen(pre);
+      if (f.compare(0, n, pre) != 0 || f.back() != ']') return false;
+      fac.push_back(type | (uint32_t)atoi(f.substr(n, f.size() - n - 1).c_str()));
+      return true;
+    };
+    if (!(idx("RCT[", F_RCT) || idx("V[", F_V) || idx("F[", F_F))) {
+      char* end = nullptr;
+      double v = strtod(f.c_str(), &end);
+      if (f.empty() || end != f.c_str() + f.size() || f[0] == '-') return false;
+      size_t k = 0;
+      while (k < pool.size() && !(pool[k] == v)) k++;
+      if (k == pool.size()) pool.push_back(v);
+      fac.push_back(F_C | (uint32_t)k);
+    }
+    if (q == std::string::npos) break;
+    p = q + 1;
+  }
+  return !fac.empty();
+}
+
+struct Chain {
+  std::vector<uint64_t> steps;          // descriptors
+  std::vector<std::vector<int>> deps;   // per step: chains whose results it reads
+  int init_dep = -1;                    // chain producing the initial value (first step)
+};
+
+// List scheduling on 64 lanes; returns the table [T][64].
+std::vector<uint64_t> schedule(const std::vector<Chain>& ch, int dist, int& T) {
+  const int n = (int)ch.size();
+  std::vector<std::vector<int>> succ(n);
+  for (int c = 0; c < n; c++) {
+    std::set<int> d;
+    for (auto& v : ch[c].deps) d.insert(v.begin(), v.end());
+    if (ch[c].init_dep >= 0) d.insert(ch[c].init_dep);
+    for (int x : d) succ[x].push_back(c);
+  }
+  // longest path from a chain's start to the end of the program
+  std::vector<long> prio(n, -1);
+  std::function<long(int)> pr = [&](int c) -> long {
+    if (prio[c] >= 0) return prio[c];
+    long m = 0;
+    for (int s : succ[c]) m = std::max(m, pr(s));
+    return prio[c] = (long)ch[c].steps.size() + dist + m;
+  };
+  for (int c = 0; c < n; c++) pr(c);
+  std::vector<int> order(n);
+  for (int c = 0; c < n; c++) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return prio[a] > prio[b]; });
+  std::vector<long> fin(n, -1), lane_free(64, 0);
+  struct Place { int lane; long start; };
+  std::vector<Place> at(n);
+  long end = 0;
+  for (int c : order) {
+    if (ch[c].steps.empty()) die("coop: empty chain");
+    long e = 0;
+    if (ch[c].init_dep >= 0) {
+      if (fin[ch[c].init_dep] < 0) die("coop: chain order violates a dependency");
+      e = std::max(e, fin[ch[c].init_dep] + dist);
+    }
+    for (size_t k = 0; k < ch[c].deps.size(); k++)
+      for (int d : ch[c].deps[k]) {
+        if (fin[d] < 0) die("coop: chain order violates a dependency");
+        e = std::max(e, fin[d] + dist - (long)k);
+      }
+    // the lane that can start it first; among those free before e, the latest free
+    int best = -1;
+    long bs = 0;
+    for (int l = 0; l < 64; l++) {
+      long s = std::max(lane_free[l], e);
+      if (best < 0 || s < bs || (s == bs && lane_free[l] > lane_free[best])) {
+        best = l;
+        bs = s;
+      }
+    }
+    at[c] = {best, bs};
+    fin[c] = bs + (long)ch[c].steps.size() - 1;
+    lane_free[best] = fin[c] + 1;
+    end = std::max(end, fin[c] + 1);
+  }
+  T = (int)end;
+  std::vector<uint64_t> tab((size_t)T * 64, IDLE);
+  for (int c = 0; c < n; c++)
+    for (size_t k = 0; k < ch[c].steps.size(); k++)
+      tab[(size_t)(at[c].start + k) * 64 + at[c].lane] = ch[c].steps[k];
+  return tab;
+}
+
+// Step descriptors, 16-bit fields.
+// Sum chains (Vdot, JVS): constant-pool index of the coefficient bits 0-15,
+// source (A or B product) bits 16-31, target bits 32-47, FIRST 48, LAST 49,
+// ZERO 50 (the value is 0.0), DIAG 51 (JVS: a diagonal entry, +ghinv).
+uint64_t sum_desc(int coef, int src, int tgt, bool first, bool last, bool zero, bool diag) {
+  return (uint64_t)coef | ((uint64_t)src << 16) | ((uint64_t)tgt << 32) |
+         ((uint64_t)first << 48) | ((uint64_t)last << 49) | ((uint64_t)zero << 50) |
+         ((uint64_t)diag << 51);
+}
+// LU / solve steps: a bits 0-15, b bits 16-31, target bits 32-47, FIRST 48,
+// LAST 49, DIV 50 (acc /= [a]; else acc -= [a]*[b])
+uint64_t op_desc(int a, int b, int t, bool first, bool last, bool div) {
+  return (uint64_t)a | ((uint64_t)b << 16) | ((uint64_t)t << 32) | ((uint64_t)first << 48) |
+         ((uint64_t)last << 49) | ((uint64_t)div << 50);
+}
+
+void emit_table(std::ostream& o, const char* type, const std::string& name, size_t n,
+                const std::function<std::string(size_t)>& item) {
+  o << "static __device__ const " << type << " " << name << "[" << std::max<size_t>(n, 1)
+    << "] = {";
+  for (size_t i = 0; i < n; i++) o << (i % 8 ? " " : "\n  ") << item(i) << ",";
+  if (n == 0) o << "0";
+  o << "};\n";
+}
+
+}  // namespace coop
+
+// The cooperative programs of a mechanism as `struct kppm_<root>_coop`
+// (ENABLED = 0 when the mechanism's code has a form the tables cannot hold).
+void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
+  using namespace coop;
+  auto& S = md.S;
+  const int nv = S.nvar, nnz = S.lu_nonzero;
+  std::string why;
+  std::vector<double> pool;
+  std::vector<std::vector<uint32_t>> aprod, bprod;
+  std::map<int, int> q_of;  // reaction -> A product index
+  int maxf = 1;
+  for (size_t q = 0; q < md.a_used.size() && why.empty(); q++) {
+    std::vector<uint32_t> f;
+    if (!parse_product(md.a_expr[q], f, pool)) why = "A product " + md.a_expr[q];
+    aprod.push_back(f);
+    q_of[md.a_used[q]] = (int)q;
+    maxf = std::max(maxf, (int)f.size());
+  }
+  for (size_t k = 0; k < md.b_expr.size() && why.empty(); k++) {
+    std::vector<uint32_t> f;
+    if (!parse_product(md.b_expr[k], f, pool)) why = "B product " + md.b_expr[k];
+    bprod.push_back(f);
+    maxf = std::max(maxf, (int)f.size());
+  }
+  const int na = (int)aprod.size(), nb = (int)bprod.size();
+  if (nnz >= 0xFFFF || 2 * nv >= 0xFFFF || na >= 0xFFFF || nb >= 0xFFFF)
+    why = "sizes beyond 16-bit indices";
+  auto pool_index = [&](double v) {
+    size_t k = 0;
+    while (k < pool.size() && !(pool[k] == v)) k++;
+    if (k == pool.size()) pool.push_back(v);
+    return (int)k;
+  };
+  std::vector<Chain> fun, jac, lu, sol;
+  // Fun: Vdot[i] chains over A products
+  for (int i = 0; i < nv && why.empty(); i++) {
+    std::vector<Term> t;
+    if (!parse_chain(md.vdot_expr[i], "A", t)) { why = "Vdot " + md.vdot_expr[i]; break; }
+    Chain c;
+    if (t.empty()) {
+      c.steps.push_back(sum_desc(0, 0, i, true, true, true, false));
+      c.deps.push_back({});
+    }
+    for (size_t k = 0; k < t.size(); k++) {
+      auto it = q_of.find(t[k].src);
+      if (it == q_of.end()) { why = "Vdot term"; break; }
+      c.steps.push_back(sum_desc(pool_index(t[k].coef), it->second, i, k == 0, k + 1 == t.size(),
+                                 false, false));
+      c.deps.push_back({});
+    }
+    fun.push_back(c);
+  }
+  // Jac_SP fused with the Ghimj build: G[n] = -(JVS chain) (+ghinv on the diagonal)
+  std::vector<char> is_diag(nnz, 0);
+  for (int i = 0; i < nv; i++) is_diag[S.lu_diag[i]] = 1;
+  for (int n = 0; n < nnz && why.empty(); n++) {
+    std::vector<Term> t;
+    if (!parse_chain(md.jvs_expr[n], "B", t)) { why = "JVS " + md.jvs_expr[n]; break; }
+    Chain c;
+    if (t.empty()) {
+      c.steps.push_back(sum_desc(0, 0, n, true, true, true, is_diag[n]));
+      c.deps.push_back({});
+    }
+    for (size_t k = 0; k < t.size(); k++) {
+      c.steps.push_back(sum_desc(pool_index(t[k].coef), t[k].src, n, k == 0, k + 1 == t.size(),
+                                 false, is_diag[n]));
+      c.deps.push_back({});
+    }
+    jac.push_back(c);
+  }
+  // KppDecomp (util/sutil.c:2-23): entry x's chain = its updates by the pivot
+  // rows in increasing order, then (L part) the division by the pivot
+  if (why.empty()) {
+    std::vector<int> chain_of(nnz, -1);
+    std::vector<std::vector<uint64_t>> st(nnz);
+    std::vector<std::vector<std::vector<int>>> dp(nnz);
+    // dependencies are recorded as entries first, mapped to chains below
+    for (int k = 0; k < nv; k++) {
+      std::map<int, int> pos;
+      for (int kk = S.lu_crow[k]; kk < S.lu_crow[k + 1]; kk++) pos[S.lu_icol[kk]] = kk;
+      for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
+        const int j = S.lu_icol[kk];
+        st[kk].push_back(op_desc(S.lu_diag[j], 0, kk, false, false, true));
+        dp[kk].push_back({S.lu_diag[j]});
+        for (int jj = S.lu_diag[j] + 1; jj < S.lu_crow[j + 1]; jj++) {
+          const int x = pos.at(S.lu_icol[jj]);
+          st[x].push_back(op_desc(kk, jj, x, false, false, false));
+          dp[x].push_back({kk, jj});
+        }
+      }
+    }
+    for (int x = 0; x < nnz; x++)
+      if (!st[x].empty()) {
+        chain_of[x] = (int)lu.size();
+        lu.push_back({});
+      }
+    for (int x = 0; x < nnz; x++) {
+      if (chain_of[x] < 0) continue;
+      Chain& c = lu[chain_of[x]];
+      for (size_t k = 0; k < st[x].size(); k++) {
+        uint64_t d = st[x][k];
+        if (k == 0) d |= 1ull << 48;
+        if (k + 1 == st[x].size()) d |= 1ull << 49;
+        c.steps.push_back(d);
+        std::vector<int> dd;
+        for (int e : dp[x][k])
+          if (chain_of[e] >= 0) dd.push_back(chain_of[e]);
+        c.deps.push_back(dd);
+      }
+    }
+  }
+  // KppSolve (the emitted forward and backward chains): forward row i in
+  // place in X, backward row i from X[i] into Z[i] (b indices >= NVAR mean Z)
+  if (why.empty()) {
+    std::vector<int> fwd(nv, -1), bwd(nv, -1);
+    for (int i = 0; i < nv; i++) {
+      const int b = S.lu_crow[i], e = S.lu_diag[i];
+      if (b == e) continue;
+      Chain c;
+      for (int j = b; j < e; j++) {
+        const int col = S.lu_icol[j];
+        c.steps.push_back(op_desc(j, col, i, j == b, j + 1 == e, false));
+        c.deps.push_back(fwd[col] >= 0 ? std::vector<int>{fwd[col]} : std::vector<int>{});
+      }
+      fwd[i] = (int)sol.size();
+      sol.push_back(c);
+    }
+    for (int i = nv - 1; i >= 0; i--) {
+      const int b = S.lu_diag[i] + 1, e = S.lu_crow[i + 1];
+      Chain c;
+      c.init_dep = fwd[i];
+      for (int j = b; j < e; j++) {
+        const int col = S.lu_icol[j];
+        c.steps.push_back(op_desc(j, nv + col, nv + i, j == b, false, false));
+        c.deps.push_back({bwd[col]});
+      }
+      c.steps.push_back(op_desc(S.lu_diag[i], 0, nv + i, b == e, true, true));
+      c.deps.push_back({});
+      bwd[i] = (int)sol.size();
+      sol.push_back(c);
+    }
+  }
+  o << "// cooperative programs (one wavefront per cell; kppgen --coop)\n";
+  if (!g_coop || !why.empty()) {
+    if (g_coop)
+      fprintf(stderr, "kppgen: no cooperative programs for %s: %s\n", md.m->root.c_str(),
+              why.c_str());
+    o << "struct " << ns << "_coop {\nstatic constexpr bool ENABLED = false;\n};\n";
+    return;
+  }
+  int tf = 0, tj = 0, tl = 0, ts = 0;
+  auto tfun = schedule(fun, 2, tf), tjac = schedule(jac, 2, tj), tlu = schedule(lu, 2, tl),
+       tsol = schedule(sol, 2, ts);
+  auto hex = [](uint64_t v) {
+    char b[32];
+    snprintf(b, sizeof b, "0x%llxull", (unsigned long long)v);
+    return std::string(b);
+  };
+  auto fac_item = [](const std::vector<std::vector<uint32_t>>& P, int np) {
+    return [&P, np](size_t i) {
+      const size_t f = i / np, q = i % np;
+      const uint32_t v = f < P[q].size() ? P[q][f] : F_END;
+      return std::to_string(v) + "u";
+    };
+  };
+  // the tables live in a namespace (device variables cannot be static data
+  // members); the struct hands out their addresses
+  o << "namespace " << ns << "_coop_data {\n";
+  // product factors, factor-major: factor f of product q at [f*N + q]
+  emit_table(o, "unsigned", "APROD", (size_t)maxf * na, fac_item(aprod, na));
+  emit_table(o, "unsigned", "BPROD", (size_t)maxf * nb, fac_item(bprod, nb));
+  emit_table(o, "double", "CONSTS", pool.size(), [&](size_t i) {
+    char b[40];
+    snprintf(b, sizeof b, "%.17g", pool[i]);
+    return std::string(b);
+  });
+  emit_table(o, "unsigned long long", "FUN", tfun.size(), [&](size_t i) { return hex(tfun[i]); });
+  emit_table(o, "unsigned long long", "JAC", tjac.size(), [&](size_t i) { return hex(tjac[i]); });
+  emit_table(o, "unsigned long long", "LU", tlu.size(), [&](size_t i) { return hex(tlu[i]); });
+  emit_table(o, "unsigned long long", "SOL", tsol.size(), [&](size_t i) { return hex(tsol[i]); });
+  emit_table(o, "unsigned", "DIAG", nv, [&](size_t i) { return std::to_string(S.lu_diag[i]) + "u"; });
+  o << "}  // namespace " << ns << "_coop_data\n";
+  o << "struct " << ns << "_coop {\n";
+  o << "static constexpr bool ENABLED = true;\n";
+  o << "static constexpr int NA = " << na << ", NB = " << nb << ", MAXF = " << maxf
+    << ", NCONST = " << pool.size() << ";\n";
+  o << "static constexpr int FUN_STEPS = " << tf << ", JAC_STEPS = " << tj
+    << ", LU_STEPS = " << tl << ", SOL_STEPS = " << ts << ";\n";
+  const char* names[] = {"APROD", "BPROD", "CONSTS", "FUN", "JAC", "LU", "SOL", "DIAG"};
+  const char* types[] = {"unsigned", "unsigned", "double", "unsigned long long",
+                         "unsigned long long", "unsigned long long", "unsigned long long",
+                         "unsigned"};
+  for (int k = 0; k < 8; k++)
+    o << "static __device__ __forceinline__ const " << types[k] << "* " << names[k]
+      << "() { return " << ns << "_coop_data::" << names[k] << "; }\n";
+  o << "};\n";
+  fprintf(stderr, "kppgen: %s coop steps: Fun %d, Jac %d, LU %d, solve %d (A %d, B %d)\n",
+          md.m->root.c_str(), tf, tj, tl, ts, na, nb);
 }
 
 void emit_stats(Model& md, const std::string& path) {
@@ -2150,6 +2541,8 @@ int main(int argc, char** argv) {
     else if (a == "--no-reorder") do_reorder = false;
     else if (a == "--rates-f90") rates_f90 = true;
     else if (a == "--no-pipelined") g_emit_pipelined = false;
+    else if (a == "--pipelined") g_emit_pipelined = true;
+    else if (a == "--coop") g_coop = true;
     else if (a == "--part-ops") g_part_ops = atol(next().c_str());
     else if (a == "--lds-split") g_lds_split = atoi(next().c_str());
     else if (a == "--f90-module") f90_modules.push_back(next());
@@ -2160,7 +2553,7 @@ int main(int argc, char** argv) {
     else if (a == "-h" || a == "--help") {
       printf("usage: kppgen [--kpp-home DIR] [--root NAME] [-o OUTDIR] [--rates-f90\n"
              "              [--f90-module FILE]... [--env 'F90 statements;...']]\n"
-             "              [--part-ops N] [--lds-split N] [--no-pipelined]\n"
+             "              [--part-ops N] [--lds-split N] [--pipelined]\n"
              "              mechanism.{def,kpp}\n");
       return 0;
     } else in = a;
@@ -2180,6 +2573,24 @@ int main(int argc, char** argv) {
   Mech m;
   m.root = root;
   m.table.push_back({"???", DUMMY_SPC, ""});
+  // Commands of src/scan.l:301-340.  Those that shape the model functions
+  // are implemented or refused; those that only select KPP's other outputs
+  // (drivers, integrator files, diagnostics, other languages) change nothing
+  // the batched integrator computes and are skipped.  Anything else is an
+  // error, so a mechanism never compiles into different equations silently.
+  static const std::set<std::string> kNoEffect = {
+      "", "INTEGRATOR", "HESSIAN", "DECLARE", "STOICMAT", "STOCHASTIC", "MEX", "DUMMYINDEX",
+      "EQNTAGS", "ATOMS", "CHECK", "CHECKALL", "LOOKAT", "LOOKATALL", "MONITOR", "WRITE_ATM",
+      "WRITE_SPC", "WRITE_MAT", "LANGUAGE", "INTFILE", "DRIVER", "UPPERCASEF90", "MINVERSION",
+      "GRAPH", "ENDINLINE"};
+  // parameter commands accepted only with the value the emitter implements
+  auto param = [](const std::string& cmd, const std::string& body, const char* ok) {
+    std::string v = upper(trim(body));
+    while (!v.empty() && v.back() == ';') v = trim(v.substr(0, v.size() - 1));
+    if (v != ok)
+      die("#" + cmd + " " + v + " is not supported (only " + ok +
+          "): it changes the model functions");
+  };
   for (auto& s : sc.sections) {
     const std::string& c = s.cmd;
     if (c == "DEFVAR" || c == "DEFRAD") declare_species(m, s.body, VAR_SPC);
@@ -2189,6 +2600,29 @@ int main(int argc, char** argv) {
     else if (c == "INITVALUES") init_values(m, s.body);
     else if (c == "EQUATIONS")
       for (auto& e : split_semis(s.body)) parse_equation(m, e);
+    else if (c == "REORDER") {
+      std::string v = upper(trim(s.body));
+      if (v.rfind("OFF", 0) == 0) do_reorder = false;
+      else if (v.rfind("ON", 0) == 0) do_reorder = true;
+      else die("#REORDER " + v + ": expected ON or OFF");
+    } else if (c == "JACOBIAN") param(c, s.body, "SPARSE_LU_ROW");
+    else if (c == "FUNCTION") param(c, s.body, "AGGREGATE");
+    else if (c == "DOUBLE") param(c, s.body, "ON");
+    else if (c == "FLUX") param(c, s.body, "OFF");
+    else if (c == "AUTOREDUCE") param(c, s.body, "OFF");
+    else if (c == "FAMILIES")
+      die("#FAMILIES is not supported: family species change the model functions");
+    else if (!kNoEffect.count(c)) die("unknown or unsupported KPP command #" + c);
+  }
+  // inline code that enters the rate constants (src/gen.c Update_RCONST):
+  // the F90 rate code is translated with --rates-f90; the C contexts are not
+  // supported, and the other languages' code does not enter the C model
+  for (auto& kv : sc.inlines) {
+    const std::string& ctx = kv.first;
+    if (!rates_f90 && (ctx == "C_RCONST" || ctx == "C_RATES"))
+      die("#INLINE " + ctx + " is not supported: its C code would enter the rate constants");
+    if (rates_f90 && ctx == "F90_RATES")
+      die("#INLINE F90_RATES is not supported by the F90 rate translator");
   }
   if (m.rates.empty()) die("no equations");
   Model md;

@@ -7,7 +7,11 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "mech_ops.h"
@@ -190,7 +194,8 @@ struct Geometry {
   size_t tol_bytes, tile_bytes, rpre_bytes, total;
 };
 
-bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry& g) {
+bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry& g,
+              bool no_rpre = false) {
   const bool mixed = l && l->lu_fp32;
   const kpphip::MixedOps* mx = mixed ? mixed_of(m) : nullptr;
   if (mixed && !mx) return false;
@@ -198,9 +203,30 @@ bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry
   if (rows <= 0) return false;
   const int variant = l ? l->variant : KPP_VARIANT_PLAIN;
   if (variant != KPP_VARIANT_PLAIN && variant != KPP_VARIANT_PERSISTENT &&
-      variant != KPP_VARIANT_REGISTER)
+      variant != KPP_VARIANT_REGISTER && variant != KPP_VARIANT_COOP)
     return false;
   if (variant == KPP_VARIANT_REGISTER && (mixed || !m->fits_registers)) return false;
+  if (variant == KPP_VARIANT_COOP) {
+    // one wavefront per cell: a block of coop_rows doubles per wavefront,
+    // LU and scratch in LDS; FP64 only, no LDS mirror, 64 lanes
+    if (mixed || !m->has_coop || (l && l->lds_entries != 0) || (l && l->lanes != 0 && l->lanes != 64))
+      return false;
+    int cus = device_cus();
+    if (cus <= 0) cus = 256;
+    int occ = m->occupancy(variant, S, 0);
+    if (occ <= 0) occ = 1;
+    int nb = (l && l->nblocks > 0) ? l->nblocks : cus * occ;
+    if (nb > ncells) nb = ncells > 0 ? ncells : 1;
+    g.nblocks = nb;
+    g.refill_min = 0;
+    g.lanes = 64;
+    g.lds_entries = 0;
+    g.tol_bytes = align256(2 * (size_t)m->nvar * sizeof(double));
+    g.tile_bytes = align256((size_t)m->coop_rows(S) * sizeof(double));
+    g.rpre_bytes = no_rpre ? 0 : align256((size_t)m->nreact * (size_t)ncells * sizeof(double));
+    g.total = kHead + g.tol_bytes + g.rpre_bytes + (size_t)g.nblocks * g.tile_bytes;
+    return true;
+  }
   if (l && l->lu_fp32 != 0 && l->lu_fp32 != 1) return false;
   // LDS mirror of LU entries: persistent FP64 kernel, sizes the mechanism is built with
   const int lds = l ? l->lds_entries : 0;
@@ -242,7 +268,7 @@ bool geometry(const MechOps* m, int S, const kpp_launch* l, int ncells, Geometry
   // the register kernel keeps the working set in VGPRs: no tiles
   g.tile_bytes = (variant == KPP_VARIANT_REGISTER) ? 0 : (size_t)rows * 64 * sizeof(double);
   // persistent kernel: rate constants of every cell from a pre-pass
-  g.rpre_bytes = (variant == KPP_VARIANT_PERSISTENT)
+  g.rpre_bytes = (variant == KPP_VARIANT_PERSISTENT && !no_rpre)
                      ? align256((size_t)m->nreact * (size_t)ncells * sizeof(double)) : 0;
   g.total = kHead + g.tol_bytes + g.rpre_bytes + (size_t)g.nblocks * g.tile_bytes;
   return true;
@@ -328,10 +354,16 @@ size_t kpp_workspace_bytes(const char* mech, int method, const kpp_launch* l, in
   return g.total;
 }
 
-int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
-                       const double RPAR[20], const double* atol, const double* rtol,
-                       const kpp_launch* l, void* workspace, size_t workspace_bytes,
-                       void* stream) {
+}  // extern "C"
+
+// kpp_rosenbrock_dev, and (fixed = true) its variant on a workspace of fixed
+// size: there the persistent kernel computes each cell's rate constants when
+// the cell starts instead of in a pre-pass over all cells (same formulae,
+// same values), and its grid is cut to the wavefronts whose tiles fit.
+static int rosenbrock_impl(const char* mech, const kpp_batch* b, const int IPAR[20],
+                           const double RPAR[20], const double* atol, const double* rtol,
+                           const kpp_launch* l, void* workspace, size_t workspace_bytes,
+                           void* stream, bool fixed) {
   const MechOps* m = find_mech(mech);
   if (!m) return KPP_E_MECH;
   if (!b || !IPAR || !RPAR || !atol || !rtol || b->ncells < 0 || b->ld < b->ncells)
@@ -353,7 +385,17 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
   memset(&p, 0, sizeof p);
   if (!fill_method(IPAR[3], p.coef)) return -2;
   Geometry g;
-  if (!geometry(m, p.coef.S, l, b->ncells, g)) return KPP_E_ARG;
+  if (!geometry(m, p.coef.S, l, b->ncells, g, fixed)) return KPP_E_ARG;
+  if (fixed && g.total > workspace_bytes && g.tile_bytes > 0 &&
+      ((l ? l->variant : KPP_VARIANT_PLAIN) == KPP_VARIANT_PERSISTENT ||
+       (l ? l->variant : KPP_VARIANT_PLAIN) == KPP_VARIANT_COOP)) {
+    const size_t head = g.total - (size_t)g.nblocks * g.tile_bytes;
+    const size_t fit = workspace_bytes > head ? (workspace_bytes - head) / g.tile_bytes : 0;
+    if (fit >= 1) {
+      g.nblocks = (int)std::min<size_t>(fit, (size_t)g.nblocks);
+      g.total = head + (size_t)g.nblocks * g.tile_bytes;
+    }
+  }
   if (!workspace || workspace_bytes < g.total) return KPP_E_WORKSPACE;
   char* ws = (char*)workspace;
   double* dtol = (double*)(ws + kHead);
@@ -397,6 +439,7 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
   p.FacRej = (RPAR[5] == 0.0) ? 0.1 : RPAR[5];
   p.FacSafe = (RPAR[6] == 0.0) ? 0.9 : RPAR[6];
   p.hstart = b->hstart;
+  p.hstart_auto = (IPAR[4] == 1);
   p.rconst = b->rconst;
   p.rates_given = b->rconst != nullptr;
   // dF/dT is exactly zero when no rate constant depends on T
@@ -419,6 +462,16 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
                                           : m->launch(p, variant, g.nblocks, st);
   if (le != hipSuccess) return KPP_E_HIP;
   return 0;
+}
+
+extern "C" {
+
+int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
+                       const double RPAR[20], const double* atol, const double* rtol,
+                       const kpp_launch* l, void* workspace, size_t workspace_bytes,
+                       void* stream) {
+  return rosenbrock_impl(mech, b, IPAR, RPAR, atol, rtol, l, workspace, workspace_bytes, stream,
+                         false);
 }
 
 static int component(const char* mech, int which, int n, int ld, const double* a, const double* b,
@@ -468,22 +521,41 @@ int kpp_rates_c_dev(const char* mech, int n, int ld, const double* temp, const d
 // ---------------------------------------------------------------------------
 struct kpp_ctx {
   const MechOps* m;
-  int ncells, method, sun_mode;
+  int ncells, method, sun_mode, flags;
   double *C, *temp, *cfactor, *sun, *tstart, *tend, *hstart, *rstat;
+  double* hused;   // starting steps the last call passed in (0: RPAR[2]'s rule)
+  double* C0;      // KPP_CTX_AUTOTUNE: the batch's state before the trials
   int *ierr, *istat;
+  int* order;      // KPP_CTX_LEARNED: launch order
   void* ws;
   size_t ws_bytes;
   std::vector<double> atol, rtol;
   int vector_tol;
+  kpp_launch launch;  // launch shape of the integrations
+  bool tuned;         // KPP_CTX_AUTOTUNE: the launch shape has been chosen
+  bool carried;       // INTEGRATE_BATCH: hstart holds the previous call's Hexit
+  kpp_sched* sched;   // KPP_CTX_LEARNED
 };
 
 static void ctx_free(kpp_ctx* c) {
   if (!c) return;
-  void* ptrs[] = {c->C, c->temp, c->cfactor, c->sun, c->tstart, c->tend,
-                  c->hstart, c->rstat, c->ierr, c->istat, c->ws};
+  void* ptrs[] = {c->C, c->temp, c->cfactor, c->sun, c->tstart, c->tend, c->hstart, c->hused,
+                  c->C0, c->rstat, c->ierr, c->istat, c->order, c->ws};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  if (c->sched) kpp_sched_destroy(c->sched);
   delete c;
+}
+
+// <directory of this library>/<mech>_step_mlp.kppw, where the build puts the
+// exported scheduler weights
+static std::string default_weights(const char* mech) {
+  Dl_info info;
+  if (!dladdr((void*)&kpp_ctx_create, &info) || !info.dli_fname) return "";
+  std::string lib(info.dli_fname);
+  const size_t k = lib.find_last_of('/');
+  const std::string dir = (k == std::string::npos) ? "." : lib.substr(0, k);
+  return dir + "/" + mech + "_step_mlp.kppw";
 }
 
 int kpp_ctx_create(kpp_ctx** out, const char* mech, int ncells, int method, int flags) {
@@ -491,20 +563,30 @@ int kpp_ctx_create(kpp_ctx** out, const char* mech, int ncells, int method, int 
   *out = nullptr;
   const MechOps* m = find_mech(mech);
   if (!m) return KPP_E_MECH;
+  const int known = KPP_SUN_TOD | KPP_CTX_PERSISTENT | KPP_CTX_AUTOTUNE | KPP_CTX_LEARNED |
+                    KPP_CTX_MIXED | KPP_CTX_HSTART_AUTO;
+  if ((flags & ~known) || ((flags & KPP_CTX_AUTOTUNE) && (flags & KPP_CTX_MIXED)))
+    return KPP_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return KPP_E_NODEV;
   kpp_ctx* c = new kpp_ctx();  // value-initialised: all pointers null
   c->m = m;
   c->ncells = ncells;
   c->method = method;
+  c->flags = flags;
   c->sun_mode = (flags & KPP_SUN_TOD) ? KPP_SUN_TOD : KPP_SUN_FIXED;
   c->atol.assign(1, 1.0);
   c->rtol.assign(1, 1.0e-3);
   c->vector_tol = 0;
   const size_t n = (size_t)ncells;
-  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0, 0, 0};
-  c->ws_bytes = kpp_workspace_bytes(mech, method, &l, ncells);
+  memset(&c->launch, 0, sizeof c->launch);
+  c->launch.variant = (flags & (KPP_CTX_PERSISTENT | KPP_CTX_AUTOTUNE)) ? KPP_VARIANT_PERSISTENT
+                                                                         : KPP_VARIANT_PLAIN;
+  c->launch.lu_fp32 = (flags & KPP_CTX_MIXED) ? 1 : 0;
+  c->ws_bytes = kpp_workspace_bytes(mech, method, &c->launch, ncells);
   if (c->ws_bytes == 0) { ctx_free(c); return -2; }
+  if (flags & KPP_CTX_AUTOTUNE)
+    c->ws_bytes = std::max(c->ws_bytes, kpp_autotune_workspace_bytes(mech, method, ncells));
   bool ok = hipMalloc(&c->C, sizeof(double) * n * m->nspec) == hipSuccess &&
             hipMalloc(&c->temp, sizeof(double) * n) == hipSuccess &&
             hipMalloc(&c->cfactor, sizeof(double) * n) == hipSuccess &&
@@ -512,21 +594,54 @@ int kpp_ctx_create(kpp_ctx** out, const char* mech, int ncells, int method, int 
             hipMalloc(&c->tstart, sizeof(double) * n) == hipSuccess &&
             hipMalloc(&c->tend, sizeof(double) * n) == hipSuccess &&
             hipMalloc(&c->hstart, sizeof(double) * n) == hipSuccess &&
+            hipMalloc(&c->hused, sizeof(double) * n) == hipSuccess &&
             hipMalloc(&c->rstat, sizeof(double) * 2 * n) == hipSuccess &&
             hipMalloc(&c->ierr, sizeof(int) * n) == hipSuccess &&
             hipMalloc(&c->istat, sizeof(int) * 8 * n) == hipSuccess &&
             hipMalloc(&c->ws, c->ws_bytes) == hipSuccess;
+  if (ok && (flags & KPP_CTX_LEARNED)) ok = hipMalloc(&c->order, sizeof(int) * n) == hipSuccess;
+  if (ok && (flags & KPP_CTX_AUTOTUNE))
+    ok = hipMalloc(&c->C0, sizeof(double) * n * m->nspec) == hipSuccess;
   if (!ok) { ctx_free(c); return KPP_E_HIP; }
   if (hipMemset(c->hstart, 0, sizeof(double) * n) != hipSuccess ||
+      hipMemset(c->hused, 0, sizeof(double) * n) != hipSuccess ||
       hipMemset(c->sun, 0, sizeof(double) * n) != hipSuccess) {
     ctx_free(c);
     return KPP_E_HIP;
+  }
+  if (flags & KPP_CTX_LEARNED) {
+    const std::string w = default_weights(mech);
+    const int r = w.empty() ? KPP_E_WEIGHTS : kpp_sched_load(w.c_str(), mech, &c->sched);
+    if (r != 0) { ctx_free(c); return r; }
   }
   *out = c;
   return 0;
 }
 
 void kpp_ctx_destroy(kpp_ctx* c) { ctx_free(c); }
+
+int kpp_ctx_load_scheduler(kpp_ctx* c, const char* path) {
+  if (!c || !path || !(c->flags & KPP_CTX_LEARNED)) return KPP_E_ARG;
+  kpp_sched* s = nullptr;
+  const int r = kpp_sched_load(path, c->m->name, &s);
+  if (r != 0) return r;
+  if (c->sched) kpp_sched_destroy(c->sched);
+  c->sched = s;
+  return 0;
+}
+
+int kpp_ctx_get_launch(kpp_ctx* c, kpp_launch* l) {
+  if (!c || !l) return KPP_E_ARG;
+  *l = c->launch;
+  return 0;
+}
+
+int kpp_ctx_get_hstart(kpp_ctx* c, double* h) {
+  if (!c || !h) return KPP_E_ARG;
+  return hipMemcpy(h, c->hused, sizeof(double) * c->ncells, hipMemcpyDeviceToHost) == hipSuccess
+             ? 0
+             : KPP_E_HIP;
+}
 
 int kpp_set_state(kpp_ctx* c, const double* C, const double* temp, const double* cfactor,
                   const double* sun) {
@@ -542,8 +657,20 @@ int kpp_set_state(kpp_ctx* c, const double* C, const double* temp, const double*
   return 0;
 }
 
+__global__ static void k_fill_double(double* p, int n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// How a call's starting steps are given.
+enum HstartMode {
+  HS_RPAR,     // none: RPAR[2]'s rule for every cell
+  HS_GIVEN,    // c->hstart holds the caller's values
+  HS_CARRIED,  // INTEGRATE_BATCH: c->hstart holds the previous call's Hexit (if c->carried)
+};
+
 static int ctx_run(kpp_ctx* c, const int* IPAR, const double* RPAR, const double* atol,
-                   const double* rtol, bool use_hstart) {
+                   const double* rtol, HstartMode hs, double span) {
   kpp_batch b;
   memset(&b, 0, sizeof b);
   b.ncells = c->ncells;
@@ -555,13 +682,45 @@ static int ctx_run(kpp_ctx* c, const int* IPAR, const double* RPAR, const double
   b.sun_mode = c->sun_mode;
   b.tstart = c->tstart;
   b.tend = c->tend;
-  b.hstart = use_hstart ? c->hstart : nullptr;
+  b.hstart = (hs == HS_RPAR) ? nullptr : c->hstart;
   b.ierr = c->ierr;
   b.istat = c->istat;
   b.rstat = c->rstat;
-  kpp_launch l = {KPP_VARIANT_PLAIN, 0, 0, 0, 0, 0};
-  int r = kpp_rosenbrock_dev(c->m->name, &b, IPAR, RPAR, atol, rtol, &l, c->ws, c->ws_bytes,
-                             nullptr);
+  const int n = c->ncells, nb = (n + 255) / 256;
+  if (c->sched) {
+    // learned starting steps where the call gives none (RPAR[2] = 0, or the
+    // first INTEGRATE_BATCH of the context), and the launch order
+    const bool predict = (hs == HS_RPAR && RPAR[2] == 0.0) || (hs == HS_CARRIED && !c->carried);
+    const double hmin = std::max(RPAR[0], (double)KPP_DELTAMIN);
+    const int r = kpp_sched_run(c->sched, &b, hmin, std::max(span, hmin),
+                                predict ? c->hstart : nullptr, c->order, nullptr);
+    if (r != 0) return r;
+    if (predict) b.hstart = c->hstart;
+    b.order = c->order;
+  }
+  if (b.hstart) {
+    if (hipMemcpyAsync(c->hused, b.hstart, sizeof(double) * n, hipMemcpyDeviceToDevice, nullptr) !=
+        hipSuccess)
+      return KPP_E_HIP;
+  } else {
+    hipLaunchKernelGGL(k_fill_double, dim3(nb), dim3(256), 0, nullptr, c->hused, n, RPAR[2]);
+  }
+  if ((c->flags & KPP_CTX_AUTOTUNE) && !c->tuned) {
+    // the launch shape is chosen once, by timing candidates on this batch
+    if (hipMemcpyAsync(c->C0, c->C, sizeof(double) * (size_t)n * c->m->nspec,
+                       hipMemcpyDeviceToDevice, nullptr) != hipSuccess)
+      return KPP_E_HIP;
+    kpp_launch best;
+    float ms = 0.f;
+    int tried = 0;
+    const int r = kpp_autotune(c->m->name, &b, c->C0, IPAR, RPAR, atol, rtol, c->ws, c->ws_bytes,
+                               nullptr, &best, &ms, &tried);
+    if (r != 0) return r;
+    c->launch = best;
+    c->tuned = true;
+  }
+  int r = kpp_rosenbrock_dev(c->m->name, &b, IPAR, RPAR, atol, rtol, &c->launch, c->ws,
+                             c->ws_bytes, nullptr);
   if (hipDeviceSynchronize() != hipSuccess) return KPP_E_HIP;
   return r;
 }
@@ -578,7 +737,9 @@ int kpp_rosenbrock_batch(kpp_ctx* c, const double* tstart, const double* tend,
   if (hstart &&
       hipMemcpy(c->hstart, hstart, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
     return KPP_E_HIP;
-  return ctx_run(c, IPAR, RPAR, atol, rtol, hstart != nullptr);
+  double span = 0.0;
+  for (size_t i = 0; i < n; i++) span = std::max(span, fabs(tend[i] - tstart[i]));
+  return ctx_run(c, IPAR, RPAR, atol, rtol, hstart ? HS_GIVEN : HS_RPAR, span);
 }
 
 int kpp_get_state(kpp_ctx* c, double* C, int* ierr, int* istat, double* rstat) {
@@ -615,7 +776,9 @@ __global__ static void k_copy_hexit(double* hs, const double* rstat, int n) {
 
 // INTEGRATE (int/rosenbrock.c:94-133): non-autonomous, RPAR[2] = STEPMIN
 // (per cell: the previous call's Hexit), method fixed at context creation
-// (the reference hard-codes 5 = Rodas4), tolerances of the context.
+// (the reference hard-codes 5 = Rodas4), tolerances of the context.  With
+// KPP_CTX_LEARNED the first call starts each cell from the scheduler's step
+// (STEPMIN is still 0 there); with KPP_CTX_HSTART_AUTO from the estimate.
 int INTEGRATE_BATCH(kpp_ctx* c, double TIN, double TOUT) {
   if (!c) return KPP_E_ARG;
   int IPAR[20];
@@ -625,13 +788,15 @@ int INTEGRATE_BATCH(kpp_ctx* c, double TIN, double TOUT) {
   IPAR[0] = 0;
   IPAR[1] = c->vector_tol ? 0 : 1;
   IPAR[3] = c->method;
+  IPAR[4] = (c->flags & KPP_CTX_HSTART_AUTO) ? 1 : 0;
   const int nb = (c->ncells + 255) / 256;
   hipLaunchKernelGGL(k_fill_time, dim3(nb), dim3(256), 0, 0, c->tstart, c->tend, c->ncells, TIN,
                      TOUT);
-  int r = ctx_run(c, IPAR, RPAR, c->atol.data(), c->rtol.data(), true);
+  int r = ctx_run(c, IPAR, RPAR, c->atol.data(), c->rtol.data(), HS_CARRIED, fabs(TOUT - TIN));
   if (r != 0) return r;
   hipLaunchKernelGGL(k_copy_hexit, dim3(nb), dim3(256), 0, 0, c->hstart, c->rstat, c->ncells);
   if (hipDeviceSynchronize() != hipSuccess) return KPP_E_HIP;
+  c->carried = true;
   return 0;
 }
 
@@ -774,4 +939,3 @@ extern "C" int kpp_autotune(const char* mech, const kpp_batch* b, const double* 
   if (rc == 0 && tried == 0) rc = KPP_E_WORKSPACE;
   return rc;
 }
-

@@ -6,6 +6,13 @@
 #define KPP_SUN_TOD 1   /* SUN from Update_SUN(TIME) at every rate update */
 #define KPP_DELTAMIN 1.0e-6
 #define KPP_SMAX 6
+/* Per-cell codes with no counterpart in int/rosenbrock.c, whose reject loop
+ * (:537-619) has no bound of its own: a NaN error estimate or a non-finite
+ * step size (-9), or more than KPP_MAX_REJECT rejected attempts in a row
+ * (-10), end the cell. */
+#define KPP_IERR_NONFINITE (-9)
+#define KPP_IERR_REJECTS (-10)
+#define KPP_MAX_REJECT 100000
 
 namespace kpphip {
 
@@ -39,6 +46,7 @@ struct RosParams {
   int max_steps;
   double Roundoff, Hmin, Hmax_user, Hstart_user, FacMin, FacMax, FacRej, FacSafe;
   const double* hstart;    // [ncells] per-cell starting step or null
+  int hstart_auto;         // IPAR[4] == 1: estimate the starting step where none is given
   const double* rconst;    // [NREACT][ldc] given rate constants or null
   double* rpre;            // [NREACT][ncells] rate constants computed by a pre-pass, or null
   int rates_given;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "mech_ops.h"
+#include "ros_coop.h"
 #include "ros_device.h"
 
 namespace kpphip {
@@ -98,9 +99,53 @@ int occupancy_persistent() {
   return n;
 }
 
-template <class M, int S, bool MIX>
+// Cooperative kernel (one wavefront per cell): FP64 only, mechanisms built
+// with cooperative programs.
+template <class M, class C, int S, bool MIX>
+constexpr bool has_coop() {
+  return !MIX && C::ENABLED;
+}
+template <class M, class C, int S, bool MIX>
+hipError_t launch_coop(const RosParams& p, int nblocks, hipStream_t st) {
+  if constexpr (has_coop<M, C, S, MIX>()) {
+    const size_t lds = CoopLds<M, C>::BYTES;
+    auto k = ros_coop_kernel<M, C, S>;
+    if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+      return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(TILE), lds, st, p);
+    return hipSuccess;
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+template <class M, class C, int S, bool MIX>
+int occupancy_coop() {
+  int n = 0;
+  if constexpr (has_coop<M, C, S, MIX>()) {
+    const size_t lds = CoopLds<M, C>::BYTES;
+    auto k = ros_coop_kernel<M, C, S>;
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, TILE, lds) != hipSuccess) n = 0;
+  }
+  return n;
+}
+
+template <class M, class C, int S, bool MIX>
 hipError_t launch_s(const RosParams& p, int variant, int nblocks, hipStream_t st) {
-  if (variant == KPP_VARIANT_PERSISTENT) {
+  if (variant == KPP_VARIANT_COOP) {
+    if constexpr (has_coop<M, C, S, MIX>()) {
+      hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, p.queue);
+      if (p.rpre)
+        hipLaunchKernelGGL(ros_rates_prepass<M>, dim3((p.ncells + 255) / 256), dim3(256), 0, st,
+                           p);
+      const hipError_t e = launch_coop<M, C, S, MIX>(p, nblocks, st);
+      if (e != hipSuccess) return e;
+    } else {
+      return hipErrorInvalidValue;
+    }
+  } else if (variant == KPP_VARIANT_PERSISTENT) {
     if (p.lds_entries != 0 && p.lds_entries != KPP_LDS_SPLIT_A &&
         p.lds_entries != KPP_LDS_SPLIT_B)
       return hipErrorInvalidValue;
@@ -126,23 +171,25 @@ hipError_t launch_s(const RosParams& p, int variant, int nblocks, hipStream_t st
   return hipGetLastError();
 }
 
-template <class M, bool MIX>
+template <class M, class C, bool MIX>
 hipError_t launch_mech(const RosParams& p, int variant, int nblocks, hipStream_t st) {
   switch (p.coef.S) {
-    case 2: return launch_s<M, 2, MIX>(p, variant, nblocks, st);
-    case 3: return launch_s<M, 3, MIX>(p, variant, nblocks, st);
-    case 4: return launch_s<M, 4, MIX>(p, variant, nblocks, st);
-    case 6: return launch_s<M, 6, MIX>(p, variant, nblocks, st);
+    case 2: return launch_s<M, C, 2, MIX>(p, variant, nblocks, st);
+    case 3: return launch_s<M, C, 3, MIX>(p, variant, nblocks, st);
+    case 4: return launch_s<M, C, 4, MIX>(p, variant, nblocks, st);
+    case 6: return launch_s<M, C, 6, MIX>(p, variant, nblocks, st);
   }
   return hipErrorInvalidValue;
 }
 
 // Resident wavefronts per CU of a kernel (its register/LDS limits).
-template <class M, int S, bool MIX>
+template <class M, class C, int S, bool MIX>
 int occupancy_s(int variant, int lds) {
   int n = 0;
   hipError_t e;
-  if (variant == KPP_VARIANT_PERSISTENT) {
+  if (variant == KPP_VARIANT_COOP) {
+    return occupancy_coop<M, C, S, MIX>();
+  } else if (variant == KPP_VARIANT_PERSISTENT) {
     if (lds == KPP_LDS_SPLIT_A) return occupancy_persistent<M, S, MIX, KPP_LDS_SPLIT_A>();
     if (lds == KPP_LDS_SPLIT_B) return occupancy_persistent<M, S, MIX, KPP_LDS_SPLIT_B>();
     if (lds != 0) return 0;
@@ -157,13 +204,24 @@ int occupancy_s(int variant, int lds) {
   }
   return e == hipSuccess ? n : 0;
 }
-template <class M, bool MIX>
+template <class M, class C, bool MIX>
 int occupancy_mech(int variant, int S, int lds) {
   switch (S) {
-    case 2: return occupancy_s<M, 2, MIX>(variant, lds);
-    case 3: return occupancy_s<M, 3, MIX>(variant, lds);
-    case 4: return occupancy_s<M, 4, MIX>(variant, lds);
-    case 6: return occupancy_s<M, 6, MIX>(variant, lds);
+    case 2: return occupancy_s<M, C, 2, MIX>(variant, lds);
+    case 3: return occupancy_s<M, C, 3, MIX>(variant, lds);
+    case 4: return occupancy_s<M, C, 4, MIX>(variant, lds);
+    case 6: return occupancy_s<M, C, 6, MIX>(variant, lds);
+  }
+  return 0;
+}
+
+template <class M>
+int coop_rows_mech(int S) {
+  switch (S) {
+    case 2: return CoopLayout<M, 2>::ROWS;
+    case 3: return CoopLayout<M, 3>::ROWS;
+    case 4: return CoopLayout<M, 4>::ROWS;
+    case 6: return CoopLayout<M, 6>::ROWS;
   }
   return 0;
 }
@@ -198,7 +256,13 @@ hipError_t component_mech(int which, int n, size_t ld, const double* a, const do
   return hipGetLastError();
 }
 
-template <class M, class D>
+template <class M, class C>
+constexpr size_t coop_lds_bytes() {
+  if constexpr (C::ENABLED) return CoopLds<M, C>::BYTES;
+  else return 0;
+}
+
+template <class M, class D, class C>
 MechOps make_ops(const char* name) {
   MechOps o;
   o.name = name;
@@ -220,9 +284,12 @@ MechOps make_ops(const char* name) {
   o.lu_crow = D::crow();
   o.lu_diag = D::diag();
   o.init = D::init();
+  o.has_coop = C::ENABLED ? 1 : 0;
+  o.coop_lds = coop_lds_bytes<M, C>();
   o.rows = rows_mech<M, false>;
-  o.occupancy = occupancy_mech<M, false>;
-  o.launch = launch_mech<M, false>;
+  o.coop_rows = coop_rows_mech<M>;
+  o.occupancy = occupancy_mech<M, C, false>;
+  o.launch = launch_mech<M, C, false>;
   return o;
 }
 
@@ -240,16 +307,18 @@ MechOps make_ops(const char* name) {
   };                                                                                    \
   }                                                                                     \
   extern "C" const kpphip::MechOps* kpphip_mech_##NAME() {                              \
-    static const kpphip::MechOps ops = kpphip::make_ops<kppm_##NAME, Data_##NAME>(#NAME); \
+    static const kpphip::MechOps ops =                                                  \
+        kpphip::make_ops<kppm_##NAME, Data_##NAME, kppm_##NAME##_coop>(#NAME);          \
     return &ops;                                                                        \
   }
 
 // Defines kpphip_mech_mixed_<NAME>(): the mixed-precision integrator of NAME.
 #define KPPHIP_DEFINE_MECH_MIXED(NAME)                                                  \
   extern "C" const kpphip::MixedOps* kpphip_mech_mixed_##NAME() {                       \
-    static const kpphip::MixedOps ops = {kpphip::rows_mech<kppm_##NAME, true>,          \
-                                         kpphip::occupancy_mech<kppm_##NAME, true>,     \
-                                         kpphip::launch_mech<kppm_##NAME, true>};       \
+    static const kpphip::MixedOps ops = {                                               \
+        kpphip::rows_mech<kppm_##NAME, true>,                                           \
+        kpphip::occupancy_mech<kppm_##NAME, kppm_##NAME##_coop, true>,                  \
+        kpphip::launch_mech<kppm_##NAME, kppm_##NAME##_coop, true>};                    \
     return &ops;                                                                        \
   }
 

@@ -8,6 +8,7 @@
 #define KPP_VARIANT_PLAIN 0
 #define KPP_VARIANT_PERSISTENT 1
 #define KPP_VARIANT_REGISTER 2
+#define KPP_VARIANT_COOP 3
 
 // LDS mirror sizes (LU entries per lane) the persistent kernel is built with,
 // where the mechanism's LDS_MAX allows (24 x 64 lanes x 8 B = 12 KiB per
@@ -30,11 +31,14 @@ struct MechOps {
   int nvar, nfix, nspec, nreact, nonzero, lu_nonzero;
   int rates_use_time, n_sun_rates, rates_use_state, n_state_rates;
   int fits_registers;  // KPP_VARIANT_REGISTER is compiled for this mechanism
+  int has_coop;        // KPP_VARIANT_COOP is compiled for this mechanism
+  size_t coop_lds;     // LDS bytes per cell of the cooperative kernel
   int lds_max;         // largest LDS mirror (LU entries) the persistent kernel is built with
   const char* const* species;
   const int *lu_icol, *lu_crow, *lu_diag;
   const double* init;  // Initialize(): #INITVALUES times CFACTOR, [NSPEC]
   int (*rows)(int S);  // tile rows per lane for S stages
+  int (*coop_rows)(int S);  // doubles per wavefront block of the cooperative kernel
   // resident wavefronts per CU (0 if unknown) with `lds` mirrored LU entries per lane
   int (*occupancy)(int variant, int S, int lds);
   hipError_t (*launch)(const RosParams& p, int variant, int nblocks, hipStream_t st);

@@ -24,6 +24,7 @@
 // are the only source of difference.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <float.h>
 #include <stdint.h>
 
 #include <utility>
@@ -41,12 +42,16 @@
 #ifndef KPPHIP_MODEL_INLINE
 #define KPPHIP_MODEL_INLINE 0
 #endif
-//  KPPHIP_PIPELINED     1: the integrator uses the memory-pipelined
-//                       KppDecompPF/KppSolvePF emitted by the compiler.
-//                       Off: with it, a persistent-kernel parity run stopped
-//                       making progress on the GPU for a reason not yet found.
+//  KPPHIP_PIPELINED     must be 0.  The memory-pipelined KppDecompPF /
+//                       KppSolvePF (kppgen --pipelined) are bitwise the plain
+//                       routines on the host, but with them a persistent-kernel
+//                       parity run stopped making progress on the GPU for a
+//                       reason not found; they stay out of the integrator.
 #ifndef KPPHIP_PIPELINED
 #define KPPHIP_PIPELINED 0
+#endif
+#if KPPHIP_PIPELINED
+#error "KPPHIP_PIPELINED=1 is disabled: the pipelined LU stalled a persistent-kernel run"
 #endif
 #if KPPHIP_WAVES_PER_EU > 0
 #define KPPHIP_KERNEL_ATTR __attribute__((amdgpu_waves_per_eu(KPPHIP_WAVES_PER_EU)))
@@ -167,19 +172,11 @@ KPPHIP_MODEL_CALL int call_jac_ghimj(dptr V, dptr F, dptr R, double ghinv, dptr 
 }
 template <class M>
 KPPHIP_MODEL_CALL void call_decomp(dptr G) {
-#if KPPHIP_PIPELINED
-  M::KppDecompPF(Acc{G});
-#else
   M::KppDecomp(Acc{G});
-#endif
 }
 template <class M>
 KPPHIP_MODEL_CALL void call_solve(dptr G, dptr X) {
-#if KPPHIP_PIPELINED
-  M::KppSolvePF(Acc{G}, Acc{X});
-#else
   M::KppSolve(Acc{G}, Acc{X});
-#endif
 }
 // LU with the first NL mirror slots in LDS (NL <= M::LDS_MAX)
 template <class M, int NL>
@@ -333,7 +330,8 @@ struct CellState {
   int Direction;
   bool RejectLastH, RejectMoreH, newstep;
   int Nfun, Njac, Nstp, Nacc, Nrej, Ndec, Nsol, Nsng;
-  int Nfb;  // attempts that fell back from the FP32 to the FP64 factorisation
+  int Nfb;   // attempts that fell back from the FP32 to the FP64 factorisation
+  int Nrun;  // rejected attempts since the last accepted one
 };
 
 template <class M, int S, bool MIX = false, class Mem = TileMem<M>>
@@ -393,20 +391,67 @@ struct Cell {
     s.Nfun = p.stat0[0]; s.Njac = p.stat0[1]; s.Nstp = p.stat0[2]; s.Nacc = p.stat0[3];
     s.Nrej = p.stat0[4]; s.Ndec = p.stat0[5]; s.Nsol = p.stat0[6]; s.Nsng = p.stat0[7];
     s.Nfb = 0;
+    s.Nrun = 0;
     s.Tstart = p.tstart ? p.tstart[cell] : p.tstart0;
     s.Tend = p.tend ? p.tend[cell] : p.tend0;
     const double span = abs_diff_macro(s.Tend, s.Tstart);
     s.Hmax = (p.Hmax_user == 0.0) ? span : dmin(dabs(p.Hmax_user), span);
     const double hs_user = p.hstart ? p.hstart[cell] : p.Hstart_user;
-    const double Hstart = (hs_user == 0.0) ? dmax(p.Hmin, KPP_DELTAMIN) : dmin(dabs(hs_user), span);
+    double Hstart = (hs_user == 0.0) ? dmax(p.Hmin, KPP_DELTAMIN) : dmin(dabs(hs_user), span);
     s.T = s.Tstart;
+    s.Direction = (s.Tend >= s.Tstart) ? +1 : -1;
+    if (hs_user == 0.0 && p.hstart_auto) Hstart = dmax(p.Hmin, dmin(initial_step(p, s), span));
     s.Hexit = 0.0;
     s.H = dmin(Hstart, s.Hmax);
     if (dabs(s.H) <= 10.0 * p.Roundoff) s.H = KPP_DELTAMIN;
-    s.Direction = (s.Tend >= s.Tstart) ? +1 : -1;
     s.RejectLastH = false;
     s.RejectMoreH = false;
     s.newstep = true;
+  }
+
+  // Starting step estimate (no counterpart in int/rosenbrock.c, whose
+  // default start is DeltaMin = 1e-6 s; selected with IPAR[4] = 1 when no
+  // starting step is given): the algorithm of Hairer, Norsett & Wanner,
+  // Solving ODEs I, II.4 ("Starting step size"), with the error norm's
+  // weights sc_i = ATOL_i + RTOL_i |y0_i|:
+  //   d0 = ||y0||, d1 = ||f(t0, y0)||, h0 = 0.01 d0/d1 (1e-6 if either is
+  //   below 1e-5), y1 = y0 + h0 f(t0, y0), d2 = ||f(t0+h0, y1) - f(t0, y0)||/h0,
+  //   h1 = (0.01/max(d1, d2))^(1/ELO) (max(1e-6, 1e-3 h0) if that max is
+  //   below 1e-15), h = min(100 h0, h1).
+  // Two function evaluations per cell, counted in Nfun.  Uses the tile rows
+  // Fcn0, Ynew and Fcn, which the first attempt overwrites.
+  __device__ double initial_step(const RosParams& p, CellState& s) const {
+    constexpr int NVAR = M::NVAR;
+    A Y = at(L::Y), F = at(L::FIX), R = at(L::RCT), F0 = at(L::FCN0), Y1 = at(L::YNEW),
+      F1 = at(L::FCN);
+    auto sc = [&](int i, double y) {
+      return p.vector_tol ? (p.atol[i] + p.rtol[i] * dabs(y)) : (p.atol0 + p.rtol0 * dabs(y));
+    };
+    rates_at(p, s, s.T, true);
+    Mem::fun(Y, F, R, F0);
+    double d0 = 0.0, d1 = 0.0;
+    vloop<U, NVAR>([&](int i) {
+      const double w = sc(i, Y[i]), a = Y[i] / w, b = F0[i] / w;
+      d0 = d0 + a * a;
+      d1 = d1 + b * b;
+    });
+    d0 = sqrt(d0 / (double)NVAR);
+    d1 = sqrt(d1 / (double)NVAR);
+    const double h0 = (d0 < 1.0e-5 || d1 < 1.0e-5) ? 1.0e-6 : 0.01 * (d0 / d1);
+    const double dh = (double)s.Direction * h0;
+    vloop<U, NVAR>([&](int i) { Y1[i] = Y[i] + dh * F0[i]; });
+    rates_at(p, s, s.T + dh, false);
+    Mem::fun(Y1, F, R, F1);
+    double d2 = 0.0;
+    vloop<U, NVAR>([&](int i) {
+      const double c = (F1[i] - F0[i]) / sc(i, Y[i]);
+      d2 = d2 + c * c;
+    });
+    d2 = sqrt(d2 / (double)NVAR) / h0;
+    s.Nfun += 2;
+    const double dm = dmax(d1, d2);
+    const double h1 = (dm <= 1.0e-15) ? dmax(1.0e-6, 1.0e-3 * h0) : pow(0.01 / dm, 1.0 / p.coef.ELO);
+    return dmin(100.0 * h0, h1);
   }
 
   __device__ void finish(const RosParams& p, const CellState& s, int ierr, double Texit) const {
@@ -565,14 +610,19 @@ struct Cell {
         call_residual<M>(G.b, XR.b, Ki.b, RR.b);
         for (int i = 0; i < NVAR; i++) X32[i] = (float)RR[i];
         call_solve32<M>(G32.b, X32.b);
+        // the max of |d| and |x| forgets a NaN (dmax(NaN, y) = y), and an
+        // infinite correction passes inf <= 1e-5*inf: non-finite values are
+        // flagged element by element instead
         double dmx = 0.0, xmx = 0.0;
+        bool bad = false;
         for (int i = 0; i < NVAR; i++) {
           const double d = (double)X32[i], x = XR[i] + d;
           XR[i] = x;
+          bad |= !isfinite(d) || !isfinite(x);
           dmx = dmax(dmx, dabs(d));
           xmx = dmax(xmx, dabs(x));
         }
-        if (dmx <= kRefineTol * xmx) {
+        if (!bad && dmx <= kRefineTol * xmx) {
           for (int i = 0; i < NVAR; i++) Ki[i] = XR[i];
         } else {
           // the sweep did not converge (or produced a non-finite value):
@@ -626,12 +676,25 @@ struct Cell {
       s.RejectMoreH = false;
       s.H = Hnew;
       s.newstep = true;
+      s.Nrun = 0;
     } else {
       if (s.Nacc >= 1) s.Nrej++;
       if (s.RejectMoreH) Hnew = H * p.FacRej;
       s.RejectMoreH = s.RejectLastH;
       s.RejectLastH = true;
       s.H = Hnew;
+      // Bounds of the reject loop, which int/rosenbrock.c:537-619 does not
+      // have: a NaN error estimate (a NaN in the cell's inputs, rates or
+      // Jacobian) or a step size that is no longer finite would reject
+      // forever, and so would step factors of 1 (FacMin = FacRej = 1).
+      if (Err != Err || !(dabs(Hnew) <= DBL_MAX)) {
+        finish(p, s, KPP_IERR_NONFINITE, s.T);
+        return KPP_IERR_NONFINITE;
+      }
+      if (++s.Nrun > KPP_MAX_REJECT) {
+        finish(p, s, KPP_IERR_REJECTS, s.T);
+        return KPP_IERR_REJECTS;
+      }
     }
     return 0;
   }

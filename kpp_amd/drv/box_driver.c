@@ -8,18 +8,41 @@
  * (a fixed splitmix64 stream), and INTEGRATE_BATCH advances all cells over
  * each interval at once.
  *
- * usage: box_driver MECH NCELLS METHOD TSTART TEND DT [OUT]
+ * usage: box_driver MECH NCELLS METHOD TSTART TEND DT [OUT [FLAGS [TEMP CFACTOR]]]
  *   METHOD is Rosenbrock's IPAR[3] code; SUN follows the time of day
- *   (Update_SUN).  Prints one summary line; writes the final C[NSPEC][NCELLS]
- *   and IERR[NCELLS] to OUT (raw doubles, then int32) when given.
- *   Exit status 0 when every cell returned IERR = 1.
+ *   (Update_SUN).  Prints one summary line; writes the final C[NSPEC][NCELLS],
+ *   IERR[NCELLS] and the starting steps of the first interval HSTART[NCELLS]
+ *   to OUT (raw doubles, int32, doubles) when given ("-": no file).
+ *   FLAGS: "plain" (default) or a comma list of persistent, autotune,
+ *   learned, mixed, hstart_auto (the KPP_CTX_* context flags; "tuned" =
+ *   autotune,learned).  TEMP (default 270 K) and CFACTOR (default 1) are
+ *   every cell's.  Exit status 0 when every cell returned IERR = 1.
  */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "kpphip.h"
+
+static int parse_flags(const char* s) {
+  int f = 0;
+  char buf[256];
+  strncpy(buf, s, sizeof buf - 1);
+  buf[sizeof buf - 1] = 0;
+  for (char* t = strtok(buf, ","); t; t = strtok(NULL, ",")) {
+    if (!strcmp(t, "plain")) continue;
+    else if (!strcmp(t, "persistent")) f |= KPP_CTX_PERSISTENT;
+    else if (!strcmp(t, "autotune")) f |= KPP_CTX_AUTOTUNE;
+    else if (!strcmp(t, "learned")) f |= KPP_CTX_LEARNED;
+    else if (!strcmp(t, "mixed")) f |= KPP_CTX_MIXED;
+    else if (!strcmp(t, "hstart_auto")) f |= KPP_CTX_HSTART_AUTO;
+    else if (!strcmp(t, "tuned")) f |= KPP_CTX_AUTOTUNE | KPP_CTX_LEARNED;
+    else return -1;
+  }
+  return f;
+}
 
 static uint64_t splitmix64(uint64_t* s) {
   uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
@@ -30,7 +53,14 @@ static uint64_t splitmix64(uint64_t* s) {
 
 int main(int argc, char** argv) {
   if (argc < 7) {
-    fprintf(stderr, "usage: %s MECH NCELLS METHOD TSTART TEND DT [OUT]\n", argv[0]);
+    fprintf(stderr, "usage: %s MECH NCELLS METHOD TSTART TEND DT [OUT [FLAGS [TEMP CFACTOR]]]\n",
+            argv[0]);
+    return 2;
+  }
+  const int flags = argc > 8 ? parse_flags(argv[8]) : 0;
+  const double T0 = argc > 10 ? atof(argv[9]) : 270.0, CF0 = argc > 10 ? atof(argv[10]) : 1.0;
+  if (flags < 0) {
+    fprintf(stderr, "bad FLAGS\n");
     return 2;
   }
   const char* mech = argv[1];
@@ -47,7 +77,8 @@ int main(int argc, char** argv) {
   double* temp = malloc(sizeof(double) * ncells);
   double* cfac = malloc(sizeof(double) * ncells);
   int* ierr = malloc(sizeof(int) * ncells);
-  if (!C || !init || !temp || !cfac || !ierr) return 2;
+  double* h0 = malloc(sizeof(double) * ncells);
+  if (!C || !init || !temp || !cfac || !ierr || !h0) return 2;
   kpp_mech_initial(mech, init);
   uint64_t st = 12345;
   for (int c = 0; c < ncells; c++) {
@@ -59,14 +90,17 @@ int main(int argc, char** argv) {
       }
       C[(size_t)s * ncells + c] = init[s] * f;
     }
-    temp[c] = 270.0;
-    cfac[c] = 1.0;
+    temp[c] = T0;
+    cfac[c] = CF0;
   }
   kpp_ctx* ctx = NULL;
-  int r = kpp_ctx_create(&ctx, mech, ncells, method, KPP_SUN_TOD);
+  int r = kpp_ctx_create(&ctx, mech, ncells, method, KPP_SUN_TOD | flags);
   if (r == 0) r = kpp_set_state(ctx, C, temp, cfac, NULL);
   int calls = 0;
-  for (double t = tstart; r == 0 && t < tend; t += dt, calls++) r = INTEGRATE_BATCH(ctx, t, t + dt);
+  for (double t = tstart; r == 0 && t < tend; t += dt, calls++) {
+    r = INTEGRATE_BATCH(ctx, t, t + dt);
+    if (r == 0 && calls == 0) r = kpp_ctx_get_hstart(ctx, h0);
+  }
   if (r == 0) r = kpp_get_state(ctx, C, ierr, NULL, NULL);
   kpp_ctx_destroy(ctx);
   if (r != 0) {
@@ -79,12 +113,13 @@ int main(int argc, char** argv) {
   for (size_t i = 0; i < (size_t)nvar * ncells; i++) sum += C[i];
   printf("box_driver %s cells=%d intervals=%d ierr_ok=%d sum_var=%.17g\n", mech, ncells, calls, ok,
          sum);
-  if (argc > 7) {
+  if (argc > 7 && strcmp(argv[7], "-") != 0) {
     FILE* f = fopen(argv[7], "wb");
     if (!f || fwrite(C, sizeof(double), (size_t)nspec * ncells, f) != (size_t)nspec * ncells ||
-        fwrite(ierr, sizeof(int), ncells, f) != (size_t)ncells || fclose(f) != 0)
+        fwrite(ierr, sizeof(int), ncells, f) != (size_t)ncells ||
+        fwrite(h0, sizeof(double), ncells, f) != (size_t)ncells || fclose(f) != 0)
       return 2;
   }
-  free(C); free(init); free(temp); free(cfac); free(ierr);
+  free(C); free(init); free(temp); free(cfac); free(ierr); free(h0);
   return ok == ncells ? 0 : 1;
 }

@@ -301,6 +301,38 @@ class StepModel:
         return m
 
 
+WEIGHTS_MAGIC = b"KPPWGT1\0"
+
+
+def export_flat(npz_path, out_path, nspec=None):
+    """Write a trained .npz as the flat weights image C hosts load with
+    kpp_sched_load (layout in kpp_amd/csrc/sched.cpp).  numpy only: runs
+    without a GPU (the build exports the shipped weights this way)."""
+    import json
+    import struct
+    z = np.load(npz_path, allow_pickle=False)
+    mech = str(z["mech"])
+    if nspec is None:
+        nspec = json.loads((solver.GEN / f"{mech}_stats.json").read_text())["NSPEC"]
+    in_pad = int(z["in_pad"])
+    w = np.ascontiguousarray(z["w"], "<f4")
+    has_c = "centres" in z.files
+    cen = np.ascontiguousarray(z["centres"] if has_c else np.zeros(NBUCKET), "<f4")
+    hs = float(z["h_scale"]) if "h_scale" in z.files else 1.0
+    name = mech.encode()
+    if len(name) > 31:
+        raise ValueError("mechanism name too long for the weights image")
+    with open(out_path, "wb") as f:
+        f.write(WEIGHTS_MAGIC)
+        f.write(name.ljust(32, b"\0"))
+        f.write(struct.pack("<4i", nspec + 2, in_pad, w.size, 1 if has_c else 0))
+        f.write(struct.pack("<4d", float(z["cfloor"]), float(z["mu_y"]), float(z["sd_y"]), hs))
+        f.write(cen.tobytes())
+        f.write(np.ascontiguousarray(z["mu"], "<f4").tobytes())
+        f.write(np.ascontiguousarray(z["sd"], "<f4").tobytes())
+        f.write(w.tobytes())
+
+
 def bucket_centres(nstp, edges):
     """Mean step count of each bucket (the buckets of ``targets``), increasing.
 

@@ -28,6 +28,7 @@ SUN_TOD = 1
 PLAIN = 0
 PERSISTENT = 1
 REGISTER = 2  # one lane per cell, working set in VGPRs (small mechanisms only)
+COOP = 3      # one wavefront per cell, LU in LDS (large mechanisms)
 
 # IPAR[3] codes of int/rosenbrock.c:218-225
 ROS2, ROS3, ROS4, RODAS3, RODAS4 = 1, 2, 3, 4, 5
@@ -43,6 +44,9 @@ IERR_TEXT = {
     -6: "No of steps exceeds maximum bound",
     -7: "Step size too small (T + H/10 = T) or H < Roundoff",
     -8: "Matrix is repeatedly singular",
+    # this library's bounds of the reject loop (none in int/rosenbrock.c)
+    -9: "Error estimate is NaN or the step size is not finite",
+    -10: "More than 100000 rejected steps in a row",
 }
 
 

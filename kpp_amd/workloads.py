@@ -177,6 +177,20 @@ def mcm_mixed(ncells=16384, seed=5, cells=None, rtol=1e-3, atol=1.0, method=solv
                  np.zeros(20), cells, {"workload": "mcm_mixed"})
 
 
+def linear6(ncells=1, method=solver.RODAS3, rtol=1e-3, atol=1.0, temp=1.0e-3):
+    """The linear6 test mechanism (kpp_amd/mechs/linear6.kpp): every species
+    at its #INITVALUES value (1), P = 1, every rate constant TEMP (growth and
+    decay cancel: dX/dt = 0), SUN fixed at 0, [0, 3600 s]; callers set C,
+    TEMP, the times or the rate constants per test."""
+    cells = np.arange(ncells, dtype=np.int64)
+    vals, cf, _ = mech_init("linear6")
+    C = np.repeat((vals * cf)[:, None], ncells, axis=1)
+    z = np.zeros(ncells)
+    return Batch("linear6", method, C, np.full(ncells, temp), np.full(ncells, cf), z.copy(),
+                 solver.SUN_FIXED, z.copy(), z + 3600.0, np.array([atol]), np.array([rtol]),
+                 solver.default_ipar(method), np.zeros(20), cells, {"workload": "linear6"})
+
+
 WORKLOADS = {
     "small_strato_tod": small_strato_tod,
     "saprc99_day": saprc99_day,

@@ -16,11 +16,28 @@ BUILD = HERE / "build"
 _libs = {}
 
 
-def lib(mech: str, fast: bool = False) -> ctypes.CDLL:
-    key = (mech, fast)
+def build_native(mech: str, out_dir) -> Path:
+    """The timing build of the oracle for this host's CPU (gcc -O2
+    -march=native -fopenmp) from oracle/ros_ref.c and the mechanism's
+    generated C model, into out_dir; used by the CPU baseline so the timing
+    build targets the machine it runs on."""
+    import subprocess
+    gen = HERE.parent / "kpp_amd" / "_gen"
+    out = Path(out_dir) / f"libros_ref_{mech}_native.so"
+    subprocess.run(["gcc", "-O2", "-march=native", "-fPIC", "-shared", "-fopenmp", "-w",
+                    f'-DKPP_MODEL_HEADER="{mech}_model.h"', "-I", str(gen), str(HERE / "ros_ref.c"),
+                    "-o", str(out), "-lm"], check=True, capture_output=True)
+    return out
+
+
+def lib(mech: str, fast: bool = False, path=None) -> ctypes.CDLL:
+    """The oracle library of a mechanism: the parity build, the fast build
+    (fast=True), or the library at `path` (e.g. build_native's)."""
+    key = (mech, fast, str(path) if path else None)
     if key in _libs:
         return _libs[key]
-    p = BUILD / (f"libros_ref_{mech}_fast.so" if fast else f"libros_ref_{mech}.so")
+    p = Path(path) if path else BUILD / (f"libros_ref_{mech}_fast.so" if fast else
+                                         f"libros_ref_{mech}.so")
     if not p.exists():
         raise FileNotFoundError(f"{p} missing: run kpp_amd.build.build_oracle()")
     L = ctypes.CDLL(str(p))
@@ -60,9 +77,9 @@ def sizes(mech: str) -> dict:
 
 
 def integrate(batch, nthreads: int = 0, fast: bool = False, hstart=None, rconst=None,
-              IPAR=None, RPAR=None, sun_mode=None, C=None):
+              IPAR=None, RPAR=None, sun_mode=None, C=None, path=None):
     """Integrate a workloads.Batch cell by cell; returns a dict of results."""
-    L = lib(batch.mech, fast)
+    L = lib(batch.mech, fast, path)
     sz = sizes(batch.mech)
     n = batch.ncells
     Cw = np.ascontiguousarray(batch.C if C is None else C, np.float64).copy()

@@ -33,6 +33,7 @@
  * reference's documentation, order conditions, closed forms, dense LU,
  * scipy reference integrations).
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -52,6 +53,10 @@
 #define HALF 0.5
 #define DeltaMin 1.0e-6
 #define Smax 6
+/* codes beyond the reference's -1..-8 (see the reject branch below) */
+#define IERR_NONFINITE (-9)
+#define IERR_REJECTS (-10)
+#define MAX_REJECT 100000
 
 typedef struct {
   int Nfun, Njac, Nstp, Nacc, Nrej, Ndec, Nsol, Nsng;
@@ -374,6 +379,7 @@ static int RosenbrockIntegrator(ros_cell* c, double* Y, double Tstart, double Te
   double H, T, Hnew, HC, HG, Fac, Tau, Err, Yerr[KPP_NVAR];
   int Direction, ioffset, j, istage;
   int RejectLastH, RejectMoreH;
+  int Nrun = 0; /* rejected attempts since the last accepted step */
   const int ros_S = m->S;
 
   T = Tstart;
@@ -441,6 +447,7 @@ static int RosenbrockIntegrator(ros_cell* c, double* Y, double Tstart, double Te
         RejectLastH = 0;
         RejectMoreH = 0;
         H = Hnew;
+        Nrun = 0;
         break;
       } else {
         if (c->st.Nacc >= 1) c->st.Nrej++;
@@ -448,11 +455,52 @@ static int RosenbrockIntegrator(ros_cell* c, double* Y, double Tstart, double Te
         RejectMoreH = RejectLastH;
         RejectLastH = 1;
         H = Hnew;
+        /* Deviation from int/rosenbrock.c:537-619, whose reject loop has no
+         * bound: a NaN error estimate or a non-finite step size ends the
+         * cell with -9, more than MAX_REJECT rejections in a row with -10
+         * (the GPU integrator must end every cell). */
+        if (Err != Err || !(ABS(Hnew) <= DBL_MAX)) { *Texit = T; return IERR_NONFINITE; }
+        if (++Nrun > MAX_REJECT) { *Texit = T; return IERR_REJECTS; }
       }
     }
   }
   *Texit = T;
   return 1;
+}
+
+/* ---------- starting step estimate (library extension, IPAR[4] = 1) ----------
+ * Not in int/rosenbrock.c (whose default start is DeltaMin): Hairer, Norsett
+ * & Wanner, Solving ODEs I, II.4, with the weights of ros_ErrorNorm
+ * sc_i = AbsTol_i + RelTol_i |y0_i|; two function evaluations, counted. */
+static double ros_InitialStep(ros_cell* c, double T, int Direction, double* Y,
+                              const double* AbsTol, const double* RelTol, int VectorTol,
+                              double ELO) {
+  double F0[KPP_NVAR], Y1[KPP_NVAR], F1[KPP_NVAR];
+  double d0 = ZERO, d1 = ZERO, d2 = ZERO, h0, h1, dm, dh, w, a, b;
+  c->Yacc = Y;
+  FunTemplate(c, T, Y, F0);
+  for (int i = 0; i < KPP_NVAR; i++) {
+    w = VectorTol ? (AbsTol[i] + RelTol[i] * ABS(Y[i])) : (AbsTol[0] + RelTol[0] * ABS(Y[i]));
+    a = Y[i] / w;
+    b = F0[i] / w;
+    d0 = d0 + a * a;
+    d1 = d1 + b * b;
+  }
+  d0 = sqrt(d0 / (double)KPP_NVAR);
+  d1 = sqrt(d1 / (double)KPP_NVAR);
+  h0 = (d0 < 1.0e-5 || d1 < 1.0e-5) ? 1.0e-6 : 0.01 * (d0 / d1);
+  dh = (double)Direction * h0;
+  for (int i = 0; i < KPP_NVAR; i++) Y1[i] = Y[i] + dh * F0[i];
+  FunTemplate(c, T + dh, Y1, F1);
+  for (int i = 0; i < KPP_NVAR; i++) {
+    w = VectorTol ? (AbsTol[i] + RelTol[i] * ABS(Y[i])) : (AbsTol[0] + RelTol[0] * ABS(Y[i]));
+    a = (F1[i] - F0[i]) / w;
+    d2 = d2 + a * a;
+  }
+  d2 = sqrt(d2 / (double)KPP_NVAR) / h0;
+  dm = MAX(d1, d2);
+  h1 = (dm <= 1.0e-15) ? MAX(1.0e-6, 1.0e-3 * h0) : pow(0.01 / dm, ONE / ELO);
+  return MIN(100.0 * h0, h1);
 }
 
 /* ---------- Rosenbrock (:137-446) ---------- */
@@ -505,6 +553,11 @@ static int Rosenbrock(ros_cell* c, double* Y, double Tstart, double Tend, const 
     case 5: Rodas4(&m); break;
     case 7: Rodas3_1(&m); break;
     default: return -2;
+  }
+  if (RPAR[2] == ZERO && IPAR[4] == 1) {
+    double h = ros_InitialStep(c, Tstart, (Tend >= Tstart) ? +1 : -1, Y, AbsTol, RelTol,
+                               VectorTol, m.ELO);
+    Hstart = MAX(Hmin, MIN(h, ABS(Tend - Tstart)));
   }
   IERR = RosenbrockIntegrator(c, Y, Tstart, Tend, AbsTol, RelTol, &m, Autonomous, VectorTol,
                               Max_no_steps, Roundoff, Hmin, Hmax, Hstart, FacMin, FacMax, FacRej,

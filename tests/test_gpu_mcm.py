@@ -6,9 +6,17 @@ Bars:
  * MCM rate constants on the device (kpp_rates_c_dev) against the oracle's
    C translation at the same state: |rel| <= 1e-12 (device exp/pow/log10/cos
    vs glibc; parity at the libm boundary is not pinned by the reference);
+ * MCM integrations with the oracle's rate constants supplied: all eight
+   counters identical in every cell and |rel| <= 1e-10 for species above
+   ATOL/RTOL in every cell;
  * MCM integrations, device rates, against the oracle: same IERR everywhere,
-   identical step counts for >= 90 % of cells, and on those cells |rel| <=
-   1e-6 for species above ATOL/RTOL;
+   identical step counts for >= 90 % of cells, on those cells |rel| <= 1e-6,
+   and on every cell |rel| <= 1e-2 (a few RTOL: a cell whose step sequence
+   differs through the device's exp/pow differs by the integration error),
+   species above ATOL/RTOL;
+ * the mixed-precision path against the oracle (rate constants supplied) on
+   every cell of mcm_mixed (96 cells) and saprc99_uneven (1,024 cells): the
+   same IERR and |rel| <= 1e-6 for species above ATOL/RTOL;
  * mixed precision against FP64 on the same cells: every cell succeeds; for
    species above ATOL/RTOL the two runs agree to a few RTOL (max |rel| <=
    1e-2), and against an FP64 run at RTOL 1e-8 the mixed run's error is at
@@ -25,18 +33,33 @@ from oracle import ref
 pytestmark = pytest.mark.gpu
 
 
-def _run(b, gpu, **kw):
+def _run(b, gpu, rconst=None, **kw):
     import torch
     d = b.to_device(gpu)
     fb = torch.zeros(b.ncells, dtype=torch.int32, device=gpu)
+    rc = None if rconst is None else torch.from_numpy(np.ascontiguousarray(rconst)).to(gpu)
     code, ierr, istat, rstat = solver.rosenbrock(
         b.mech, d["C"], d["temp"], d["cfactor"], tstart=d["tstart"], tend=d["tend"],
         atol=b.atol, rtol=b.rtol, IPAR=b.IPAR, RPAR=b.RPAR, sun=d["sun"], sun_mode=b.sun_mode,
-        fallbacks=fb, **kw)
+        fallbacks=fb, rconst=rc, **kw)
     torch.cuda.synchronize()
     assert code == 0
     return {"C": d["C"].cpu().numpy(), "ierr": ierr.cpu().numpy(), "istat": istat.cpu().numpy(),
             "fallbacks": fb.cpu().numpy()}
+
+
+def _oracle_rates(b):
+    """rate constants of each cell at its initial state (the oracle's)"""
+    nr = ref.sizes(b.mech)["NREACT"]
+    L = ref.lib(b.mech)
+    R = np.zeros((nr, b.ncells))
+    tmp = np.zeros(nr)
+    for c in range(b.ncells):
+        Cc = np.ascontiguousarray(b.C[:, c])
+        L.ros_ref_rconst_c(b.temp[c], b.sun[c], b.cfactor[c], b.tstart[c], Cc.ctypes.data,
+                           tmp.ctypes.data)
+        R[:, c] = tmp
+    return R
 
 
 def _rel(a, b, nvar, thr):
@@ -78,6 +101,21 @@ def test_mcm_rates_match_oracle(gpu):
 
 
 @pytest.mark.parametrize("variant", [solver.PLAIN, solver.PERSISTENT])
+def test_mcm_rates_given_matches_oracle(gpu, variant):
+    b = workloads.mcm_mixed(ncells=96, seed=62)
+    R = _oracle_rates(b)
+    o = ref.integrate(b, rconst=R)
+    g = _run(b, gpu, rconst=R, variant=variant)
+    assert np.array_equal(g["ierr"], o["ierr"]) and np.all(o["ierr"] == 1)
+    assert np.array_equal(g["istat"], o["istat"])  # all 8 counters, every cell
+    nvar = solver.mechanism("mcm").nvar
+    thr = float(b.atol[0] / b.rtol[0])
+    r = _rel(g["C"], o["C"], nvar, thr)
+    print("mcm rates given: max rel", r.max())
+    assert r.max() <= 1e-10
+
+
+@pytest.mark.parametrize("variant", [solver.PLAIN, solver.PERSISTENT])
 def test_mcm_integration_matches_oracle(gpu, variant):
     b = workloads.mcm_mixed(ncells=96, seed=62)
     o = ref.integrate(b)
@@ -89,6 +127,26 @@ def test_mcm_integration_matches_oracle(gpu, variant):
     nvar = solver.mechanism("mcm").nvar
     thr = float(b.atol[0] / b.rtol[0])
     r = _rel(g["C"][:, same], o["C"][:, same], nvar, thr)
+    assert r.max() <= 1e-6
+    r_all = _rel(g["C"], o["C"], nvar, thr)
+    print("all cells: max rel", r_all.max())
+    assert r_all.max() <= 1e-2
+
+
+@pytest.mark.parametrize("wl,n", [("mcm_mixed", 96), ("saprc99_uneven", 1024)])
+@pytest.mark.parametrize("variant", [solver.PLAIN, solver.PERSISTENT])
+def test_mixed_precision_matches_oracle(gpu, wl, n, variant):
+    b = workloads.WORKLOADS[wl](ncells=n, seed=64)
+    R = _oracle_rates(b)
+    o = ref.integrate(b, rconst=R)
+    g = _run(b, gpu, rconst=R, variant=variant, lu_fp32=1)
+    assert np.array_equal(g["ierr"], o["ierr"]) and np.all(o["ierr"] == 1)
+    nvar = solver.mechanism(b.mech).nvar
+    thr = float(b.atol[0] / b.rtol[0])
+    r = _rel(g["C"], o["C"], nvar, thr)
+    same = np.all(g["istat"] == o["istat"], axis=0)
+    print(wl, "mixed vs oracle: max rel", r.max(), "identical counters", same.mean(),
+          "fallback attempts", int(g["fallbacks"].sum()))
     assert r.max() <= 1e-6
 
 

@@ -10,6 +10,7 @@ with their defining invariants.
 import ctypes
 import json
 import re
+import subprocess
 from pathlib import Path
 
 import numpy as np
@@ -253,3 +254,56 @@ def test_saprc_pattern_invariants(mech):
         for r in rows:
             F[r, k:] |= F[k, k:]
     assert (F == P).all()
+
+
+# ------------------------------------------------------ commands it refuses
+KPPGEN = Path(__file__).resolve().parents[1] / "kpp_amd" / "_obj" / "kppgen"
+
+_BASE = """#DEFVAR
+X = IGNORE;
+Y = IGNORE;
+#DEFFIX
+P = IGNORE;
+#EQUATIONS
+<R1> X + P = Y : TEMP;
+<R2> Y = X : 2.0E-3;
+"""
+
+
+def _kppgen_on(tmp_path, text, *opts):
+    if not KPPGEN.exists():
+        pytest.skip("kppgen not built")
+    src = tmp_path / "m.kpp"
+    src.write_text(text)
+    out = tmp_path / "out"
+    out.mkdir(exist_ok=True)
+    return subprocess.run([str(KPPGEN), "--root", "m", "-o", str(out), *opts, str(src)],
+                          capture_output=True, text=True)
+
+
+def test_kppgen_compiles_plain_mechanism(tmp_path):
+    r = _kppgen_on(tmp_path, _BASE + "#LOOKATALL\n#MONITOR X;\n#INTEGRATOR rosenbrock\n")
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("extra,why", [
+    ("#FAMILIES\nFX : X + Y;\n", "FAMILIES"),
+    ("#INLINE C_RCONST\n  RCONST[0] = 1.0;\n#ENDINLINE\n", "C_RCONST"),
+    ("#INLINE C_RATES\ndouble k(double t) { return t; }\n#ENDINLINE\n", "C_RATES"),
+    ("#FUNCTION SPLIT\n", "FUNCTION"),
+    ("#JACOBIAN FULL\n", "JACOBIAN"),
+    ("#DOUBLE OFF\n", "DOUBLE"),
+    ("#FLUX ON\n", "FLUX"),
+    ("#NOSUCHCOMMAND\n", "NOSUCHCOMMAND"),
+])
+def test_kppgen_refuses_what_changes_the_model(tmp_path, extra, why):
+    """Commands that would change the model functions and that kppgen does not
+    implement are errors, not silently skipped."""
+    r = _kppgen_on(tmp_path, _BASE + extra)
+    assert r.returncode != 0
+    assert why in r.stderr + r.stdout
+
+
+def test_kppgen_reorder_off(tmp_path):
+    r = _kppgen_on(tmp_path, _BASE + "#REORDER OFF\n")
+    assert r.returncode == 0, r.stderr
