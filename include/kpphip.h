@@ -142,6 +142,23 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
                        const kpp_launch* l, void* workspace, size_t workspace_bytes,
                        void* stream);
 
+/* kpp_rosenbrock_dev without a workspace argument.  No counterpart in KPP: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, hosts that forbid allocation).  It calls no hipMalloc,
+ * hipFree or other allocating routine: the workspace is a library-owned
+ * device array of KPP_STATIC_SCRATCH_BYTES per GPU, held until the process
+ * ends.  The persistent kernel computes each cell's rate constants when the
+ * cell starts (no pre-pass buffer) and runs as many wavefronts as fit; the
+ * plain kernel needs a tile per 64 cells and a batch whose tiles exceed the
+ * array returns KPP_E_WORKSPACE, as does any launch that does not fit.
+ * Results are those of kpp_rosenbrock_dev bit for bit.  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (two streams,
+ * two host threads) race: serialise them. */
+#define KPP_STATIC_SCRATCH_BYTES ((size_t)256 << 20)
+int kpp_rosenbrock_dev_static(const char* mech, const kpp_batch* b, const int IPAR[20],
+                              const double RPAR[20], const double* atol, const double* rtol,
+                              const kpp_launch* l, void* stream);
+
 /* ---- batched generated-model functions (device SoA arrays, ld >= n) ---- */
 /* Fun(V,F,RCT,Vdot)                 src/gen.c:642-760 */
 int kpp_fun_dev(const char* mech, int n, int ld, const double* V, const double* F,

@@ -939,3 +939,43 @@ extern "C" int kpp_autotune(const char* mech, const kpp_batch* b, const double* 
   if (rc == 0 && tried == 0) rc = KPP_E_WORKSPACE;
   return rc;
 }
+
+
+// ---- workspace-free entry point ----
+// `scratch` is the device memory kpp_rosenbrock_dev_static uses instead of a
+// caller's workspace: one array per GPU, held until the process ends.  Host
+// code never passes the symbol itself to a kernel (on the host it only
+// names the array): each device's address comes from hipGetSymbolAddress,
+// taken once per device with that device current.
+static __device__ __align__(256) char scratch[KPP_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+
+int scratch_of_current_device(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return KPP_E_HIP;
+  if (dev < 0 || dev >= kMaxDevices) return KPP_E_NODEV;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return KPP_E_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" int kpp_rosenbrock_dev_static(const char* mech, const kpp_batch* b, const int IPAR[20],
+                                         const double RPAR[20], const double* atol,
+                                         const double* rtol, const kpp_launch* l, void* stream) {
+  if (!find_mech(mech)) return KPP_E_MECH;
+  void* ws = nullptr;
+  const int e = scratch_of_current_device(&ws);
+  if (e != 0) return e;
+  return rosenbrock_impl(mech, b, IPAR, RPAR, atol, rtol, l, ws, KPP_STATIC_SCRATCH_BYTES, stream,
+                         true);
+}
