@@ -2130,7 +2130,12 @@ bool g_coop = false;
 
 namespace coop {
 
-constexpr uint64_t IDLE = ~0ull;
+// An idle (lane, step) is all zeros; every real step has the VALID bit (52).
+// Tables carry PAD idle steps past the last, so the device can request
+// descriptors ahead without a bound check.
+constexpr uint64_t IDLE = 0;
+constexpr uint64_t VALID = 1ull << 52;
+constexpr int PAD = 64;
 // product factors: type in bits 30-31 (0 RCT, 1 V, 2 F, 3 constant), index below
 constexpr uint32_t F_RCT = 0u << 30, F_V = 1u << 30, F_F = 2u << 30, F_C = 3u << 30;
 constexpr uint32_t F_END = 0xFFFFFFFFu;
@@ -2261,27 +2266,28 @@ std::vector<uint64_t> schedule(const std::vector<Chain>& ch, int dist, int& T) {
     end = std::max(end, fin[c] + 1);
   }
   T = (int)end;
-  std::vector<uint64_t> tab((size_t)T * 64, IDLE);
+  std::vector<uint64_t> tab((size_t)(T + PAD) * 64, IDLE);
   for (int c = 0; c < n; c++)
     for (size_t k = 0; k < ch[c].steps.size(); k++)
       tab[(size_t)(at[c].start + k) * 64 + at[c].lane] = ch[c].steps[k];
   return tab;
 }
 
-// Step descriptors, 16-bit fields.
-// Sum chains (Vdot, JVS): constant-pool index of the coefficient bits 0-15,
-// source (A or B product) bits 16-31, target bits 32-47, FIRST 48, LAST 49,
-// ZERO 50 (the value is 0.0), DIAG 51 (JVS: a diagonal entry, +ghinv).
+// Step descriptors, 16-bit fields, VALID bit 52.
+// Sum chains (Vdot, JVS): constant-pool index of the coefficient bits 0-15
+// (the coefficient itself is also tabled per step), source (A or B product)
+// bits 16-31, target bits 32-47, FIRST 48, LAST 49, ZERO 50 (the value is
+// 0.0), DIAG 51 (JVS: a diagonal entry, +ghinv).
 uint64_t sum_desc(int coef, int src, int tgt, bool first, bool last, bool zero, bool diag) {
   return (uint64_t)coef | ((uint64_t)src << 16) | ((uint64_t)tgt << 32) |
          ((uint64_t)first << 48) | ((uint64_t)last << 49) | ((uint64_t)zero << 50) |
-         ((uint64_t)diag << 51);
+         ((uint64_t)diag << 51) | VALID;
 }
 // LU / solve steps: a bits 0-15, b bits 16-31, target bits 32-47, FIRST 48,
 // LAST 49, DIV 50 (acc /= [a]; else acc -= [a]*[b])
 uint64_t op_desc(int a, int b, int t, bool first, bool last, bool div) {
   return (uint64_t)a | ((uint64_t)b << 16) | ((uint64_t)t << 32) | ((uint64_t)first << 48) |
-         ((uint64_t)last << 49) | ((uint64_t)div << 50);
+         ((uint64_t)last << 49) | ((uint64_t)div << 50) | VALID;
 }
 
 void emit_table(std::ostream& o, const char* type, const std::string& name, size_t n,
@@ -2453,19 +2459,29 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
     snprintf(b, sizeof b, "0x%llxull", (unsigned long long)v);
     return std::string(b);
   };
-  auto fac_item = [](const std::vector<std::vector<uint32_t>>& P, int np) {
-    return [&P, np](size_t i) {
-      const size_t f = i / np, q = i % np;
-      const uint32_t v = f < P[q].size() ? P[q][f] : F_END;
+  // product tables: factor-major, each factor row padded to a multiple of
+  // 256 products with end codes (so four products per lane load unguarded)
+  const int na_pad = (na + 255) / 256 * 256, nb_pad = (nb + 255) / 256 * 256;
+  auto fac_item = [](const std::vector<std::vector<uint32_t>>& P, int npad) {
+    return [&P, npad](size_t i) {
+      const size_t f = i / npad, q = i % npad;
+      const uint32_t v = (q < P.size() && f < P[q].size()) ? P[q][f] : F_END;
       return std::to_string(v) + "u";
+    };
+  };
+  auto coef_of = [&](const std::vector<uint64_t>& tab) {
+    return [&tab, &pool](size_t i) {
+      char b[40];
+      snprintf(b, sizeof b, "%.17g", (tab[i] && !pool.empty()) ? pool[tab[i] & 0xFFFF] : 0.0);
+      return std::string(b);
     };
   };
   // the tables live in a namespace (device variables cannot be static data
   // members); the struct hands out their addresses
   o << "namespace " << ns << "_coop_data {\n";
   // product factors, factor-major: factor f of product q at [f*N + q]
-  emit_table(o, "unsigned", "APROD", (size_t)maxf * na, fac_item(aprod, na));
-  emit_table(o, "unsigned", "BPROD", (size_t)maxf * nb, fac_item(bprod, nb));
+  emit_table(o, "unsigned", "APROD", (size_t)maxf * na_pad, fac_item(aprod, na_pad));
+  emit_table(o, "unsigned", "BPROD", (size_t)maxf * nb_pad, fac_item(bprod, nb_pad));
   emit_table(o, "double", "CONSTS", pool.size(), [&](size_t i) {
     char b[40];
     snprintf(b, sizeof b, "%.17g", pool[i]);
@@ -2473,6 +2489,8 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   });
   emit_table(o, "unsigned long long", "FUN", tfun.size(), [&](size_t i) { return hex(tfun[i]); });
   emit_table(o, "unsigned long long", "JAC", tjac.size(), [&](size_t i) { return hex(tjac[i]); });
+  emit_table(o, "double", "FUN_COEF", tfun.size(), coef_of(tfun));
+  emit_table(o, "double", "JAC_COEF", tjac.size(), coef_of(tjac));
   emit_table(o, "unsigned long long", "LU", tlu.size(), [&](size_t i) { return hex(tlu[i]); });
   emit_table(o, "unsigned long long", "SOL", tsol.size(), [&](size_t i) { return hex(tsol[i]); });
   emit_table(o, "unsigned", "DIAG", nv, [&](size_t i) { return std::to_string(S.lu_diag[i]) + "u"; });
@@ -2480,14 +2498,16 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   o << "struct " << ns << "_coop {\n";
   o << "static constexpr bool ENABLED = true;\n";
   o << "static constexpr int NA = " << na << ", NB = " << nb << ", MAXF = " << maxf
-    << ", NCONST = " << pool.size() << ";\n";
+    << ", NCONST = " << pool.size() << ", NA_PAD = " << na_pad << ", NB_PAD = " << nb_pad
+    << ", PAD = " << PAD << ";\n";
   o << "static constexpr int FUN_STEPS = " << tf << ", JAC_STEPS = " << tj
     << ", LU_STEPS = " << tl << ", SOL_STEPS = " << ts << ";\n";
-  const char* names[] = {"APROD", "BPROD", "CONSTS", "FUN", "JAC", "LU", "SOL", "DIAG"};
+  const char* names[] = {"APROD", "BPROD", "CONSTS", "FUN", "JAC", "LU", "SOL", "DIAG",
+                         "FUN_COEF", "JAC_COEF"};
   const char* types[] = {"unsigned", "unsigned", "double", "unsigned long long",
                          "unsigned long long", "unsigned long long", "unsigned long long",
-                         "unsigned"};
-  for (int k = 0; k < 8; k++)
+                         "unsigned", "double", "double"};
+  for (int k = 0; k < 10; k++)
     o << "static __device__ __forceinline__ const " << types[k] << "* " << names[k]
       << "() { return " << ns << "_coop_data::" << names[k] << "; }\n";
   o << "};\n";

@@ -57,25 +57,179 @@ struct CoopLds {
   static constexpr size_t BYTES = sizeof(double) * (size_t)(M::LU_NONZERO + TSIZE);
 };
 
-// Plain pointer accessor for the emitted rate routines.
-struct PAcc {
+// Global-memory accessor for the emitted rate routines (stride 1).
+struct Acc1 {
   using value_type = double;
-  double* b;
-  __device__ __forceinline__ double& operator[](int i) const { return b[i]; }
+  gdouble* b;
+  __device__ __forceinline__ gdouble& operator[](int i) const { return b[i]; }
 };
 
-constexpr unsigned long long kIdle = ~0ull;
 constexpr unsigned kFacEnd = 0xFFFFFFFFu;
+
+// Typed address spaces matter here: through generic pointers every access
+// to LDS is a flat access, which the hardware serialises with global memory
+// (each wait for an LDS value becomes a wait for every outstanding load).
+using gcdouble = const __attribute__((address_space(1))) double;
+using gcu64 = const __attribute__((address_space(1))) unsigned long long;
+using gcu32 = const __attribute__((address_space(1))) unsigned;
+
+// ---- the program interpreters, out of line: each is a small loop with few
+// live values, so the register allocator keeps its prefetch ring and
+// operands in registers whatever the integrator around it holds.
+
+// Descriptors of the next RING steps are requested ahead (a rotating ring:
+// slot k holds step base+k until step base+k runs, then step base+k+RING);
+// the tables carry idle steps past their end, so requests need no bound.
+#ifndef KPPHIP_COOP_RING
+#define KPPHIP_COOP_RING 8
+#endif
+constexpr int kRing = KPPHIP_COOP_RING;
+
+// Product factor value: RCT[j], V[i], F[i] or a constant, without a branch
+// (one load from the base the factor's type selects); a slot past the
+// product's last factor is 1.0, and x*1.0 == x exactly.
+__device__ __forceinline__ double coop_factor(unsigned code, gcdouble* V, gcdouble* R,
+                                              gcdouble* F, gcdouble* K) {
+  const unsigned idx = (code == kFacEnd) ? 0u : (code & 0x3FFFFFFFu), type = code >> 30;
+  gcdouble* base = (type == 0) ? R : (type == 1) ? V : (type == 2) ? F : K;
+  const double v = base[idx];
+  return code == kFacEnd ? 1.0 : v;
+}
+// T[q] = product q (A of Fun or B of Jac_SP) at state V, left to right; four
+// products per lane at a time (the tables are padded to whole rounds)
+template <int MAXF, int NPAD>
+__device__ __noinline__ void coop_products(const unsigned* Pg, int N, gcdouble* V, gcdouble* R,
+                                           gcdouble* F, gcdouble* K, ldouble* T, int lane) {
+  gcu32* P = (gcu32*)Pg;
+  constexpr int PU = 4;
+  static_assert(NPAD % (TILE * PU) == 0, "product tables are padded to 256");
+  for (int q0 = lane; q0 < N; q0 += TILE * PU) {
+    unsigned code[PU][MAXF];
+#pragma unroll
+    for (int u = 0; u < PU; u++)
+#pragma unroll
+      for (int f = 0; f < MAXF; f++) code[u][f] = P[f * NPAD + q0 + u * TILE];
+    double v[PU][MAXF];
+#pragma unroll
+    for (int u = 0; u < PU; u++)
+#pragma unroll
+      for (int f = 0; f < MAXF; f++) v[u][f] = coop_factor(code[u][f], V, R, F, K);
+#pragma unroll
+    for (int u = 0; u < PU; u++) {
+      double pr = v[u][0];
+#pragma unroll
+      for (int f = 1; f < MAXF; f++) pr = pr * v[u][f];
+      const int q = q0 + u * TILE;
+      if (q < N) T[q] = pr;
+    }
+  }
+}
+
+// A sum program (Vdot or JVS chains over the products in T).  At a chain's
+// end the value goes to out[t] (JVS: G[t] = -value, +ghinv on the diagonal).
+// Idle steps (descriptor 0) read T[0] and store nothing.
+template <bool JVS, class Out>
+__device__ __noinline__ void coop_sums(const unsigned long long* progg, const double* coefg,
+                                       int steps, ldouble* T, Out* out, double ghinv, int lane) {
+  gcu64* prog = (gcu64*)progg;
+  gcdouble* cf = (gcdouble*)coefg;
+  unsigned long long r[kRing];
+  double c[kRing];
+#pragma unroll
+  for (int k = 0; k < kRing; k++) {
+    r[k] = prog[k * TILE + lane];
+    c[k] = cf[k * TILE + lane];
+  }
+  double acc = 0.0;
+  for (int base = 0; base < steps; base += kRing) {
+#pragma unroll
+    for (int k = 0; k < kRing; k++) {
+      const unsigned long long d = r[k];
+      const double coef = c[k];
+      const int tn = (base + k + kRing) * TILE + lane;
+      r[k] = prog[tn];
+      c[k] = cf[tn];
+      const double x = T[(d >> 16) & 0xFFFF];
+      const double pr = coef * x;
+      const double term = ((d >> 50) & 1) ? 0.0 : pr;
+      const double sum = acc + term;
+      acc = ((d >> 48) & 1) ? term : sum;
+      if ((d >> 49) & 1) {
+        double v = acc;
+        if (JVS) {
+          v = -(acc);
+          if ((d >> 51) & 1) v = v + ghinv;
+        }
+        out[(d >> 32) & 0xFFFF] = v;
+      }
+    }
+  }
+}
+
+// An elimination program over LDS: LU (X = G) or the triangular solves
+// (X = T, forward results in X[0:NVAR), backward in X[NVAR:2NVAR)).
+// Step: FIRST: acc = X[init(t)]; DIV: acc = acc / G[a], else
+// acc = acc - G[a]*X[b]; LAST: X[t] = acc.  The LDS operands of step t+1 are
+// read (unconditionally; idle steps read entry 0) before step t stores.
+template <int NVAR, bool SOLVE>
+__device__ __noinline__ void coop_program(const unsigned long long* progg, int steps, ldouble* G,
+                                          ldouble* X, int lane) {
+  gcu64* prog = (gcu64*)progg;
+  auto init_of = [](unsigned t) {
+    // backward solve chains start from the forward result X[i] (t = NVAR+i)
+    return (SOLVE && t >= (unsigned)NVAR) ? t - (unsigned)NVAR : t;
+  };
+  unsigned long long r[kRing];
+#pragma unroll
+  for (int k = 0; k < kRing; k++) r[k] = prog[k * TILE + lane];
+  double ga = G[r[0] & 0xFFFF], xb = X[(r[0] >> 16) & 0xFFFF],
+         xt = X[init_of((r[0] >> 32) & 0xFFFF)], acc = 0.0;
+  for (int base = 0; base < steps; base += kRing) {
+#pragma unroll
+    for (int k = 0; k < kRing; k++) {
+      const unsigned long long d = r[k];
+      r[k] = prog[(base + k + kRing) * TILE + lane];
+      // step base+k+1: slot k+1, or (k = RING-1) slot 0, refilled at k = 0
+      const unsigned long long dn = r[(k + 1) % kRing];
+      const double gan = G[dn & 0xFFFF], xbn = X[(dn >> 16) & 0xFFFF],
+                   xtn = X[init_of((dn >> 32) & 0xFFFF)];
+      if ((d >> 48) & 1) acc = xt;
+      if ((d >> 50) & 1) acc = acc / ga;
+      else acc = acc - ga * xb;
+      if ((d >> 49) & 1) X[(d >> 32) & 0xFFFF] = acc;
+      ga = gan;
+      xb = xbn;
+      xt = xtn;
+    }
+  }
+}
+
+// The emitted rate routines, out of line (lane 0 runs them).
+template <class M>
+__device__ __noinline__ void coop_rates_fixed(double TEMP, double CF, gdouble* R) {
+  M::rates_fixed(TEMP, CF, Acc1{R});
+}
+template <class M>
+__device__ __noinline__ void coop_rates_sun(double TEMP, double CF, double SUN, double TIME,
+                                            gdouble* R) {
+  M::rates_sun(TEMP, CF, SUN, TIME, Acc1{R});
+}
+template <class M>
+__device__ __noinline__ void coop_rates_state(double TEMP, double CF, double SUN, double TIME,
+                                              gdouble* V, gdouble* F, gdouble* R) {
+  M::rates_state(TEMP, CF, SUN, TIME, Acc1{V}, Acc1{F}, Acc1{R});
+}
 
 template <class M, class C, int S>
 struct CoopCell {
   using L = CoopLayout<M, S>;
-  double* w;   // this wavefront's block in global memory
-  double* G;   // LDS: LU entries
-  double* T;   // LDS: scratch
+  gdouble* w;  // this wavefront's block in global memory
+  ldouble* G;  // LDS: LU entries
+  ldouble* T;  // LDS: scratch
   int lane;
 
-  __device__ __forceinline__ double* row(int r) const { return w + r; }
+  __device__ __forceinline__ gdouble* row(int r) const { return w + r; }
+  __device__ __forceinline__ static gcdouble* cst() { return (gcdouble*)C::CONSTS(); }
 
   // all lanes: f(i) for the species this lane holds
   template <class F>
@@ -85,69 +239,27 @@ struct CoopCell {
   // every lane sees the others' stores (global memory and LDS)
   __device__ __forceinline__ static void sync() { __syncthreads(); }
 
-  // product factor value: RCT[j], V[i], F[i] or a constant
-  __device__ __forceinline__ double factor(unsigned code, const double* V) const {
-    const unsigned idx = code & 0x3FFFFFFFu;
-    switch (code >> 30) {
-      case 0: return row(L::RCT)[idx];
-      case 1: return V[idx];
-      case 2: return row(L::FIX)[idx];
-      default: return C::CONSTS()[idx];
-    }
-  }
-  // T[q] = product q (A of Fun or B of Jac_SP) at state V, left to right
-  __device__ __forceinline__ void products(const unsigned* P, int N, const double* V) const {
-    for (int q = lane; q < N; q += TILE) {
-      double p = factor(P[q], V);
-      for (int f = 1; f < C::MAXF; f++) {
-        const unsigned code = P[f * N + q];
-        if (code == kFacEnd) break;
-        p = p * factor(code, V);
-      }
-      T[q] = p;
-    }
-  }
-
   // Fun(V) -> out (a global row): products, then the Vdot chains
-  __device__ void fun(const double* V, double* out) const {
-    products(C::APROD(), C::NA, V);
+  __device__ __forceinline__ void fun(gcdouble* V, gdouble* out) const {
+    coop_products<C::MAXF, C::NA_PAD>(C::APROD(), C::NA, V, (gcdouble*)row(L::RCT),
+                                      (gcdouble*)row(L::FIX), cst(), T, lane);
     sync();
-    const unsigned long long* prog = C::FUN();
-    const double* cst = C::CONSTS();
-    double acc = 0.0;
-    for (int t = 0; t < C::FUN_STEPS; t++) {
-      const unsigned long long d = prog[t * TILE + lane];
-      if (d == kIdle) continue;
-      const double term = ((d >> 50) & 1) ? 0.0 : cst[d & 0xFFFF] * T[(d >> 16) & 0xFFFF];
-      acc = ((d >> 48) & 1) ? term : acc + term;
-      if ((d >> 49) & 1) out[(d >> 32) & 0xFFFF] = acc;
-    }
+    coop_sums<false>(C::FUN(), C::FUN_COEF(), C::FUN_STEPS, T, out, 0.0, lane);
     sync();
   }
 
   // Jac_SP at (Y, rates) fused with the Ghimj build into G (LDS): G = -JVS,
   // G[diag] += ghinv.  Returns the first row (1-based) whose diagonal is
   // exactly zero (the KppDecomp test of util/sutil.c:8), or 0.
-  __device__ int jac_ghimj(double ghinv) const {
-    products(C::BPROD(), C::NB, row(L::Y));
+  __device__ __forceinline__ int jac_ghimj(double ghinv) const {
+    coop_products<C::MAXF, C::NB_PAD>(C::BPROD(), C::NB, (gcdouble*)row(L::Y),
+                                      (gcdouble*)row(L::RCT), (gcdouble*)row(L::FIX), cst(), T,
+                                      lane);
     sync();
-    const unsigned long long* prog = C::JAC();
-    const double* cst = C::CONSTS();
-    double acc = 0.0;
-    for (int t = 0; t < C::JAC_STEPS; t++) {
-      const unsigned long long d = prog[t * TILE + lane];
-      if (d == kIdle) continue;
-      const double term = ((d >> 50) & 1) ? 0.0 : cst[d & 0xFFFF] * T[(d >> 16) & 0xFFFF];
-      acc = ((d >> 48) & 1) ? term : acc + term;
-      if ((d >> 49) & 1) {
-        double g = -(acc);
-        if ((d >> 51) & 1) g = g + ghinv;
-        G[(d >> 32) & 0xFFFF] = g;
-      }
-    }
+    coop_sums<true>(C::JAC(), C::JAC_COEF(), C::JAC_STEPS, T, G, ghinv, lane);
     sync();
     int ising = 0x7FFFFFFF;
-    const unsigned* dg = C::DIAG();
+    gcu32* dg = (gcu32*)C::DIAG();
     for (int i = lane; i < M::NVAR; i += TILE)
       if (G[dg[i]] == 0.0 && i + 1 < ising) ising = i + 1;
     for (int off = 32; off > 0; off >>= 1) {
@@ -157,52 +269,20 @@ struct CoopCell {
     return ising == 0x7FFFFFFF ? 0 : ising;
   }
 
-  // An elimination program over LDS: LU (X = G) or the solves (X = T).
-  // Step: FIRST: acc = X[init(t)]; DIV: acc = acc / G[a], else
-  // acc = acc - G[a]*X[b]; LAST: X[t] = acc.
-  template <bool SOLVE>
-  __device__ __forceinline__ void program(const unsigned long long* prog, int steps) const {
-    double* X = SOLVE ? T : G;
-    auto init_of = [](unsigned t) {
-      // backward solve chains start from the forward result X[i] (t = NVAR+i)
-      return (SOLVE && t >= (unsigned)M::NVAR) ? t - (unsigned)M::NVAR : t;
-    };
-    unsigned long long d = steps > 0 ? prog[lane] : kIdle;
-    double ga = 0.0, xb = 0.0, xt = 0.0;
-    if (d != kIdle) {
-      ga = G[d & 0xFFFF];
-      if (!((d >> 50) & 1)) xb = X[(d >> 16) & 0xFFFF];
-      if ((d >> 48) & 1) xt = X[init_of((d >> 32) & 0xFFFF)];
-    }
-    double acc = 0.0;
-    for (int t = 0; t < steps; t++) {
-      // operands of the next step, read before this step's store
-      const unsigned long long dn = (t + 1 < steps) ? prog[(t + 1) * TILE + lane] : kIdle;
-      double gan = 0.0, xbn = 0.0, xtn = 0.0;
-      if (dn != kIdle) {
-        gan = G[dn & 0xFFFF];
-        if (!((dn >> 50) & 1)) xbn = X[(dn >> 16) & 0xFFFF];
-        if ((dn >> 48) & 1) xtn = X[init_of((dn >> 32) & 0xFFFF)];
-      }
-      if (d != kIdle) {
-        if ((d >> 48) & 1) acc = xt;
-        acc = ((d >> 50) & 1) ? acc / ga : acc - ga * xb;
-        if ((d >> 49) & 1) X[(d >> 32) & 0xFFFF] = acc;
-      }
-      d = dn;
-      ga = gan;
-      xb = xbn;
-      xt = xtn;
-    }
+  __device__ __forceinline__ void decomp() const {
+    coop_program<M::NVAR, false>(C::LU(), C::LU_STEPS, G, G, lane);
     sync();
   }
-  __device__ void decomp() const { program<false>(C::LU(), C::LU_STEPS); }
   // solve G x = T[0:NVAR) in place; the result is left in T[NVAR:2NVAR)
-  __device__ void solve() const { program<true>(C::SOL(), C::SOL_STEPS); }
+  __device__ __forceinline__ void solve() const {
+    coop_program<M::NVAR, true>(C::SOL(), C::SOL_STEPS, G, T, lane);
+    sync();
+  }
 
   // rate constants at time T (see Cell::rates_at); the serial rate code runs
   // on lane 0, which writes the global RCT row
-  __device__ void rates_at(const RosParams& p, const CellState& s, double Tm, bool accepted) const {
+  __device__ __forceinline__ void rates_at(const RosParams& p, const CellState& s, double Tm,
+                                           bool accepted) const {
     if (p.rates_given) return;
     const bool tod = p.sun_mode == KPP_SUN_TOD;
     const bool sun_rates = tod || M::RATES_USE_TIME;
@@ -210,22 +290,43 @@ struct CoopCell {
     if (!sun_rates && !state_rates) return;
     if (lane == 0) {
       const double sun = tod ? update_sun(Tm) : s.SUN;
-      if (sun_rates) M::rates_sun(s.TEMP, s.CFACTOR, sun, Tm, PAcc{row(L::RCT)});
+      if (sun_rates) coop_rates_sun<M>(s.TEMP, s.CFACTOR, sun, Tm, row(L::RCT));
       if (state_rates)
-        M::rates_state(s.TEMP, s.CFACTOR, sun, Tm, PAcc{row(L::Y)}, PAcc{row(L::FIX)},
-                       PAcc{row(L::RCT)});
+        coop_rates_state<M>(s.TEMP, s.CFACTOR, sun, Tm, row(L::Y), row(L::FIX), row(L::RCT));
     }
     sync();
   }
 
-  __device__ void start(const RosParams& p, int cell, CellState& s) const {
+  // ---- the integrator; the lane's own species (i = lane + 64 r) are kept
+  // in registers (Regs), the rows of the block are what other lanes gather
+  static constexpr int NR = (M::NVAR + TILE - 1) / TILE;
+  struct Regs {
+    double y[NR], f0[NR], fc[NR], dfdt[NR], k[S][NR];
+  };
+  template <class F>
+  __device__ __forceinline__ void own(F&& f) const {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int i = lane + r * TILE;
+      if (i < M::NVAR) f(r, i);
+    }
+  }
+  __device__ __forceinline__ void load_own(gdouble* rowp, double* v) const {
+    own([&](int r, int i) { v[r] = rowp[i]; });
+  }
+
+  __device__ __forceinline__ void start(const RosParams& p, int cell, CellState& s,
+                                        Regs& g) const {
     const size_t ld = (size_t)p.ldc;
     s.cell = cell;
     s.TEMP = p.temp[cell];
     s.CFACTOR = p.cfactor[cell];
     s.SUN = p.sun ? p.sun[cell] : 0.0;
-    double *Y = row(L::Y), *F = row(L::FIX), *R = row(L::RCT);
-    for (int i = lane; i < M::NVAR; i += TILE) Y[i] = p.C[i * ld + cell];
+    gdouble *Y = row(L::Y), *F = row(L::FIX), *R = row(L::RCT);
+    own([&](int r, int i) {
+      g.y[r] = p.C[i * ld + cell];
+      Y[i] = g.y[r];
+    });
     for (int i = lane; i < M::NFIX; i += TILE) F[i] = p.C[(M::NVAR + i) * ld + cell];
     if (p.rates_given) {
       for (int i = lane; i < M::NREACT; i += TILE) R[i] = p.rconst[i * ld + cell];
@@ -236,9 +337,9 @@ struct CoopCell {
       for (int i = lane; i < M::NREACT; i += TILE) R[i] = 0.0;  // NUMBER-rate slots
       sync();
       if (lane == 0) {
-        M::rates_fixed(s.TEMP, s.CFACTOR, PAcc{R});
+        coop_rates_fixed<M>(s.TEMP, s.CFACTOR, R);
         if (p.sun_mode == KPP_SUN_FIXED && !M::RATES_USE_TIME)
-          M::rates_sun(s.TEMP, s.CFACTOR, s.SUN, 0.0, PAcc{R});
+          coop_rates_sun<M>(s.TEMP, s.CFACTOR, s.SUN, 0.0, R);
       }
     }
     s.Nfun = p.stat0[0]; s.Njac = p.stat0[1]; s.Nstp = p.stat0[2]; s.Nacc = p.stat0[3];
@@ -254,7 +355,8 @@ struct CoopCell {
     s.T = s.Tstart;
     s.Direction = (s.Tend >= s.Tstart) ? +1 : -1;
     sync();
-    if (hs_user == 0.0 && p.hstart_auto) Hstart = dmax(p.Hmin, dmin(initial_step(p, s), span));
+    if (hs_user == 0.0 && p.hstart_auto)
+      Hstart = dmax(p.Hmin, dmin(initial_step(p, s, g), span));
     s.Hexit = 0.0;
     s.H = dmin(Hstart, s.Hmax);
     if (dabs(s.H) <= 10.0 * p.Roundoff) s.H = KPP_DELTAMIN;
@@ -264,30 +366,36 @@ struct CoopCell {
   }
 
   // sum of T[0:n) in index order, computed by lane 0 and seen by all lanes
-  __device__ double ordered_sum(int n) const {
+  __device__ __forceinline__ double ordered_sum(int n) const {
     double acc = 0.0;
     if (lane == 0)
       for (int i = 0; i < n; i++) acc = acc + T[i];
     return __shfl(acc, 0);
   }
 
+  // Fun of the row V into the registers v (the chains write the row OUT)
+  __device__ __forceinline__ void fun_to(gcdouble* V, int out_row, double* v) const {
+    fun(V, row(out_row));
+    load_own(row(out_row), v);
+  }
+
   // the starting step estimate of Cell::initial_step (IPAR[4] = 1)
-  __device__ double initial_step(const RosParams& p, CellState& s) const {
-    double *Y = row(L::Y), *F0 = row(L::FCN0), *Y1 = row(L::YNEW), *F1 = row(L::FCN);
+  __device__ __forceinline__ double initial_step(const RosParams& p, CellState& s,
+                                                 Regs& g) const {
     auto sc = [&](int i, double y) {
       return p.vector_tol ? (p.atol[i] + p.rtol[i] * dabs(y)) : (p.atol0 + p.rtol0 * dabs(y));
     };
     rates_at(p, s, s.T, true);
-    fun(Y, F0);
-    vec([&](int i) {
-      const double w = sc(i, Y[i]), a = Y[i] / w;
+    fun_to((gcdouble*)row(L::Y), L::FCN0, g.f0);
+    own([&](int r, int i) {
+      const double w = sc(i, g.y[r]), a = g.y[r] / w;
       T[i] = a * a;
     });
     sync();
     double d0 = ordered_sum(M::NVAR);
     sync();
-    vec([&](int i) {
-      const double b = F0[i] / sc(i, Y[i]);
+    own([&](int r, int i) {
+      const double b = g.f0[r] / sc(i, g.y[r]);
       T[i] = b * b;
     });
     sync();
@@ -296,12 +404,14 @@ struct CoopCell {
     d1 = sqrt(d1 / (double)M::NVAR);
     const double h0 = (d0 < 1.0e-5 || d1 < 1.0e-5) ? 1.0e-6 : 0.01 * (d0 / d1);
     const double dh = (double)s.Direction * h0;
-    vec([&](int i) { Y1[i] = Y[i] + dh * F0[i]; });
+    gdouble* Y1 = row(L::YNEW);
+    own([&](int r, int i) { Y1[i] = g.y[r] + dh * g.f0[r]; });
     sync();
     rates_at(p, s, s.T + dh, false);
-    fun(Y1, F1);
-    vec([&](int i) {
-      const double c = (F1[i] - F0[i]) / sc(i, Y[i]);
+    double f1[NR];
+    fun_to((gcdouble*)Y1, L::FCN, f1);
+    own([&](int r, int i) {
+      const double c = (f1[r] - g.f0[r]) / sc(i, g.y[r]);
       T[i] = c * c;
     });
     sync();
@@ -315,11 +425,11 @@ struct CoopCell {
     return dmin(100.0 * h0, h1);
   }
 
-  __device__ void finish(const RosParams& p, const CellState& s, int ierr, double Texit) const {
+  __device__ __forceinline__ void finish(const RosParams& p, const CellState& s, const Regs& g,
+                                         int ierr, double Texit) const {
     const size_t ld = (size_t)p.ldc;
     const int cell = s.cell;
-    const double* Y = row(L::Y);
-    for (int i = lane; i < M::NVAR; i += TILE) p.C[i * ld + cell] = Y[i];
+    own([&](int r, int i) { p.C[i * ld + cell] = g.y[r]; });
     if (lane == 0) {
       p.ierr[cell] = ierr;
       if (p.istat) {
@@ -338,38 +448,35 @@ struct CoopCell {
   }
 
   // One step attempt (Cell::attempt's control flow and arithmetic).
-  __device__ int attempt(const RosParams& p, CellState& s) const {
+  __device__ __forceinline__ int attempt(const RosParams& p, CellState& s, Regs& g) const {
     constexpr int NVAR = M::NVAR;
     const RosCoef& m = p.coef;
     const double Roundoff = p.Roundoff;
-    double *Y = row(L::Y), *Fcn0 = row(L::FCN0), *Fcn = row(L::FCN), *dFdT = row(L::DFDT),
-           *Ynew = row(L::YNEW), *K0 = row(L::K);
     const bool skip_dfdt = p.dfdt_zero;
     const double Dir = (double)s.Direction;
 
     if (s.newstep) {
       const bool more = ((s.Direction > 0) && ((s.T - s.Tend) + Roundoff <= 0.0)) ||
                         ((s.Direction < 0) && ((s.Tend - s.T) + Roundoff <= 0.0));
-      if (!more) { finish(p, s, 1, s.T); return 1; }
-      if (s.Nstp > p.max_steps) { finish(p, s, -6, s.T); return -6; }
-      if (((s.T + 0.1 * s.H) == s.T) || (s.H <= Roundoff)) { finish(p, s, -7, s.T); return -7; }
+      if (!more) { finish(p, s, g, 1, s.T); return 1; }
+      if (s.Nstp > p.max_steps) { finish(p, s, g, -6, s.T); return -6; }
+      if (((s.T + 0.1 * s.H) == s.T) || (s.H <= Roundoff)) { finish(p, s, g, -7, s.T); return -7; }
       s.Hexit = s.H;
       s.H = dmin(s.H, abs_diff_macro(s.Tend, s.T));
       rates_at(p, s, s.T, true);
-      fun(Y, Fcn0);
+      fun_to((gcdouble*)row(L::Y), L::FCN0, g.f0);
       s.Nfun++;
       if (!p.autonomous) {
         s.Nfun++;
         if (!skip_dfdt) {
           const double Delta = sqrt(Roundoff) * dmax(KPP_DELTAMIN, dabs(s.T));
           rates_at(p, s, s.T + Delta, false);
-          fun(Y, dFdT);
+          fun_to((gcdouble*)row(L::Y), L::DFDT, g.dfdt);
           const double inv = 1.0 / Delta;
-          vec([&](int i) {
-            const double d = dFdT[i] + (-1.0) * Fcn0[i];
-            dFdT[i] = (inv == 1.0) ? d : inv * d;
+          own([&](int r, int i) {
+            const double d = g.dfdt[r] + (-1.0) * g.f0[r];
+            g.dfdt[r] = (inv == 1.0) ? d : inv * d;
           });
-          sync();
         }
       }
       s.Njac++;
@@ -388,13 +495,14 @@ struct CoopCell {
       if (Nconsecutive <= 5) {
         s.H = s.H * 0.5;
       } else {
-        finish(p, s, -8, s.T);
+        finish(p, s, g, -8, s.T);
         return -8;
       }
     }
     decomp();
 
     const double H = s.H;
+    gdouble* Ynew = row(L::YNEW);
     auto stage = [&](auto stage_no) __attribute__((always_inline)) {
       constexpr int istage = decltype(stage_no)::value;
       bool from_fcn0 = true;
@@ -402,22 +510,21 @@ struct CoopCell {
       for (int q = 2; q <= istage; q++)
         if (m.NewF[q - 1]) from_fcn0 = false;
       if (istage > 1 && m.NewF[istage - 1]) {
-        vec([&](int i) {
-          double y = Y[i];
+        own([&](int r, int i) {
+          double y = g.y[r];
 #pragma unroll
           for (int j = 1; j <= istage - 1; j++) {
             const double a = m.A[(istage - 1) * (istage - 2) / 2 + j - 1];
-            if (a != 0.0) y = y + a * K0[(j - 1) * NVAR + i];
+            if (a != 0.0) y = y + a * g.k[j - 1][r];
           }
           Ynew[i] = y;
         });
         sync();
         const double Tau = s.T + m.Alpha[istage - 1] * Dir * H;
         rates_at(p, s, Tau, false);
-        fun(Ynew, Fcn);
+        fun_to((gcdouble*)Ynew, L::FCN, g.fc);
         s.Nfun++;
       }
-      const double* Fc = from_fcn0 ? Fcn0 : Fcn;
       double HC[S > 1 ? S - 1 : 1];
 #pragma unroll
       for (int j = 1; j <= istage - 1; j++)
@@ -426,34 +533,34 @@ struct CoopCell {
       const bool use_g = (!p.autonomous) && (m.Gamma[istage - 1] != 0.0) && !skip_dfdt &&
                          (HG != 0.0);
       // the stage's right-hand side into T[0:NVAR), solved in LDS
-      vec([&](int i) {
-        double k = Fc[i];
+      own([&](int r, int i) {
+        double k = from_fcn0 ? g.f0[r] : g.fc[r];
 #pragma unroll
         for (int j = 1; j <= istage - 1; j++)
-          if (HC[j - 1] != 0.0) k = k + HC[j - 1] * K0[(j - 1) * NVAR + i];
-        if (use_g) k = k + HG * dFdT[i];
+          if (HC[j - 1] != 0.0) k = k + HC[j - 1] * g.k[j - 1][r];
+        if (use_g) k = k + HG * g.dfdt[r];
         T[i] = k;
       });
       sync();
       solve();
-      double* Ki = K0 + (istage - 1) * NVAR;
-      vec([&](int i) { Ki[i] = T[NVAR + i]; });
-      sync();
+      own([&](int r, int i) { g.k[istage - 1][r] = T[NVAR + i]; });
       s.Nsol++;
     };
     stages(stage, std::make_integer_sequence<int, S>{});
     // new solution, error estimate and ros_ErrorNorm (int/rosenbrock.c:582-591, :632-659)
-    vec([&](int i) {
-      const double y = Y[i];
-      double yn = y, ye = 0.0;
+    double yn[NR];
+    sync();
+    own([&](int r, int i) {
+      const double y = g.y[r];
+      double ynr = y, ye = 0.0;
 #pragma unroll
       for (int j = 1; j <= S; j++) {
-        const double kj = K0[(j - 1) * NVAR + i];
-        if (m.M[j - 1] != 0.0) yn = yn + m.M[j - 1] * kj;
+        const double kj = g.k[j - 1][r];
+        if (m.M[j - 1] != 0.0) ynr = ynr + m.M[j - 1] * kj;
         if (m.E[j - 1] != 0.0) ye = ye + m.E[j - 1] * kj;
       }
-      Ynew[i] = yn;
-      const double Ymax = dmax(dabs(y), dabs(yn));
+      yn[r] = ynr;
+      const double Ymax = dmax(dabs(y), dabs(ynr));
       const double Scale =
           p.vector_tol ? (p.atol[i] + p.rtol[i] * Ymax) : (p.atol0 + p.rtol0 * Ymax);
       T[i] = (ye * ye) / (Scale * Scale);
@@ -468,7 +575,11 @@ struct CoopCell {
     if ((Err <= 1.0) || (H <= p.Hmin)) {
       if (p.hfirst && s.Nacc == p.stat0[3] && lane == 0) p.hfirst[s.cell] = H;
       s.Nacc++;
-      vec([&](int i) { Y[i] = Ynew[i]; });
+      gdouble* Y = row(L::Y);
+      own([&](int r, int i) {
+        g.y[r] = yn[r];
+        Y[i] = yn[r];
+      });
       sync();
       s.T += Dir * H;
       Hnew = dmax(p.Hmin, dmin(Hnew, s.Hmax));
@@ -485,11 +596,11 @@ struct CoopCell {
       s.RejectLastH = true;
       s.H = Hnew;
       if (Err != Err || !(dabs(Hnew) <= DBL_MAX)) {
-        finish(p, s, KPP_IERR_NONFINITE, s.T);
+        finish(p, s, g, KPP_IERR_NONFINITE, s.T);
         return KPP_IERR_NONFINITE;
       }
       if (++s.Nrun > KPP_MAX_REJECT) {
-        finish(p, s, KPP_IERR_REJECTS, s.T);
+        finish(p, s, g, KPP_IERR_REJECTS, s.T);
         return KPP_IERR_REJECTS;
       }
     }
@@ -499,21 +610,35 @@ struct CoopCell {
 
 // Persistent grid of one-wavefront workgroups; each takes the next cell of
 // the queue until the queue is exhausted (every wavefront reaches the exit).
+// Register budget: as many wavefronts per SIMD as the LDS lets a compute
+// unit hold (160 KiB over the cell's LDS bytes, four SIMDs), at most 8.
+#ifndef KPPHIP_COOP_MAX_WAVES
+#define KPPHIP_COOP_MAX_WAVES 2
+#endif
+template <class M, class C>
+constexpr int coop_waves_per_eu() {
+  const int wgs = (int)((160u * 1024u) / CoopLds<M, C>::BYTES);
+  const int w = wgs / 4;
+  return w < 1 ? 1 : (w > KPPHIP_COOP_MAX_WAVES ? KPPHIP_COOP_MAX_WAVES : w);
+}
+
 template <class M, class C, int S>
-__global__ __launch_bounds__(64) void ros_coop_kernel(RosParams p) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(coop_waves_per_eu<M, C>())))
+void ros_coop_kernel(RosParams p) {
   using L = CoopLayout<M, S>;
   extern __shared__ double coop_lds[];
   const int lane = threadIdx.x;
-  CoopCell<M, C, S> c{p.scratch + (size_t)blockIdx.x * L::ROWS, coop_lds,
-                      coop_lds + M::LU_NONZERO, lane};
+  CoopCell<M, C, S> c{(gdouble*)(p.scratch + (size_t)blockIdx.x * L::ROWS), (ldouble*)coop_lds,
+                      (ldouble*)coop_lds + M::LU_NONZERO, lane};
   for (;;) {
     int k = 0;
     if (lane == 0) k = atomicAdd(p.queue, 1);
     k = __shfl(k, 0);
     if (k >= p.ncells) break;
     CellState s;
-    c.start(p, p.order ? p.order[k] : k, s);
-    while (c.attempt(p, s) == 0) {
+    typename CoopCell<M, C, S>::Regs g;
+    c.start(p, p.order ? p.order[k] : k, s, g);
+    while (c.attempt(p, s, g) == 0) {
     }
   }
 }

@@ -33,6 +33,7 @@ using std::acos; using std::fabs;
 
 PROG = r"""#include "%(mech)s_hip.h"
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 struct HA {
   using value_type = double;
@@ -41,7 +42,7 @@ struct HA {
 };
 using M = kppm_%(mech)s;
 using C = kppm_%(mech)s_coop;
-static const unsigned long long IDLE = ~0ull;
+static const unsigned long long IDLE = 0;  // idle (lane, step); real steps have bit 52
 extern "C" int enabled() { return C::ENABLED ? 1 : 0; }
 
 // ---- reference: the straight-line routines ----
@@ -66,41 +67,50 @@ static double factor(unsigned code, const double* V, const double* F, const doub
     default: return C::CONSTS()[idx];
   }
 }
-static void products(const unsigned* P, int N, const double* V, const double* F, const double* R,
-                     double* T) {
+static void products(const unsigned* P, int N, int NPAD, const double* V, const double* F,
+                     const double* R, double* T) {
   for (int q = 0; q < N; q++) {
     double p = factor(P[q], V, F, R);
     for (int f = 1; f < C::MAXF; f++) {
-      const unsigned code = P[f * N + q];
+      const unsigned code = P[f * NPAD + q];
       if (code == 0xFFFFFFFFu) break;
       p = p * factor(code, V, F, R);
     }
     T[q] = p;
   }
 }
-// sum chains in lockstep; store(lane-independent) = out[target] = f(acc)
+// sum chains in lockstep; the coefficient comes from the per-step table
+// (which must agree with the constant pool index of the descriptor)
 template <class Store>
-static void sums(const unsigned long long* prog, int steps, const double* T, Store store) {
+static void sums(const unsigned long long* prog, const double* coef, int steps, const double* T,
+                 Store store) {
   double acc[64] = {0};
   for (int t = 0; t < steps; t++)
     for (int l = 0; l < 64; l++) {
       const unsigned long long d = prog[t * 64 + l];
       if (d == IDLE) continue;
-      const double term = ((d >> 50) & 1) ? 0.0 : C::CONSTS()[d & 0xFFFF] * T[(d >> 16) & 0xFFFF];
+      if (!((d >> 52) & 1)) abort();
+      const double c = coef[t * 64 + l];
+      if (!(c == C::CONSTS()[d & 0xFFFF]) && !((d >> 50) & 1)) abort();
+      const double term = ((d >> 50) & 1) ? 0.0 : c * T[(d >> 16) & 0xFFFF];
       acc[l] = ((d >> 48) & 1) ? term : acc[l] + term;
       if ((d >> 49) & 1) store(d, acc[l]);
     }
+  // the padding past the last step is idle
+  for (int t = steps; t < steps + C::PAD; t++)
+    for (int l = 0; l < 64; l++)
+      if (prog[t * 64 + l] != IDLE) abort();
 }
 extern "C" void coop_fun(double* V, double* F, double* R, double* out) {
   static double T[C::NA > 0 ? C::NA : 1];
-  products(C::APROD(), C::NA, V, F, R, T);
-  sums(C::FUN(), C::FUN_STEPS, T,
+  products(C::APROD(), C::NA, C::NA_PAD, V, F, R, T);
+  sums(C::FUN(), C::FUN_COEF(), C::FUN_STEPS, T,
        [&](unsigned long long d, double a) { out[(d >> 32) & 0xFFFF] = a; });
 }
 extern "C" void coop_jac(double* V, double* F, double* R, double ghinv, double* G) {
   static double T[C::NB > 0 ? C::NB : 1];
-  products(C::BPROD(), C::NB, V, F, R, T);
-  sums(C::JAC(), C::JAC_STEPS, T, [&](unsigned long long d, double a) {
+  products(C::BPROD(), C::NB, C::NB_PAD, V, F, R, T);
+  sums(C::JAC(), C::JAC_COEF(), C::JAC_STEPS, T, [&](unsigned long long d, double a) {
     double g = -(a);
     if ((d >> 51) & 1) g = g + ghinv;
     G[(d >> 32) & 0xFFFF] = g;
@@ -113,12 +123,12 @@ static void program(const unsigned long long* prog, int steps, double* G, double
   auto init_of = [&](unsigned t) { return (solve && t >= (unsigned)nvar) ? t - nvar : t; };
   double acc[64] = {0}, ga[64] = {0}, xb[64] = {0}, xt[64] = {0};
   unsigned long long d[64];
+  // every lane reads its three operands each step, idle ones entry 0
   auto load = [&](int t, int l, double& a, double& b, double& c) {
-    const unsigned long long e = (t < steps) ? prog[t * 64 + l] : IDLE;
-    if (e == IDLE) return e;
+    const unsigned long long e = prog[t * 64 + l];
     a = G[e & 0xFFFF];
-    if (!((e >> 50) & 1)) b = X[(e >> 16) & 0xFFFF];
-    if ((e >> 48) & 1) c = X[init_of((e >> 32) & 0xFFFF)];
+    b = X[(e >> 16) & 0xFFFF];
+    c = X[init_of((e >> 32) & 0xFFFF)];
     return e;
   };
   for (int l = 0; l < 64; l++) d[l] = load(0, l, ga[l], xb[l], xt[l]);
@@ -127,7 +137,7 @@ static void program(const unsigned long long* prog, int steps, double* G, double
     double gan[64] = {0}, xbn[64] = {0}, xtn[64] = {0};
     for (int l = 0; l < 64; l++) dn[l] = load(t + 1, l, gan[l], xbn[l], xtn[l]);
     for (int l = 0; l < 64; l++) {
-      if (d[l] == IDLE) continue;
+      if (d[l] != IDLE && !((d[l] >> 52) & 1)) abort();
       if ((d[l] >> 48) & 1) acc[l] = xt[l];
       acc[l] = ((d[l] >> 50) & 1) ? acc[l] / ga[l] : acc[l] - ga[l] * xb[l];
       if ((d[l] >> 49) & 1) X[(d[l] >> 32) & 0xFFFF] = acc[l];

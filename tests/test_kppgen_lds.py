@@ -141,3 +141,54 @@ def test_reaction_major_fun_bitwise(host_lu):
         R.ros_ref_fun(_p(V), _p(F), _p(K), _p(b))
         assert np.array_equal(a, b)
 
+
+
+# ---- the memory-pipelined LU (kppgen --pipelined; not used by the integrator)
+PF_PROG = """#include "%(mech)s_hip.h"
+struct HA {
+  using value_type = double;
+  double* b;
+  double& operator[](int i) const { return b[i]; }
+};
+using M = kppm_%(mech)s;
+extern "C" void ref_lu(double* G, double* X) { M::KppDecomp(HA{G}); M::KppSolve(HA{G}, HA{X}); }
+extern "C" void pf_lu(double* G, double* X) { M::KppDecompPF(HA{G}); M::KppSolvePF(HA{G}, HA{X}); }
+"""
+
+
+@pytest.mark.parametrize("mech", ["saprc99", "saprcnov"])
+def test_pipelined_lu_bitwise_on_reused_buffer(tmp_path, mech):
+    """KppDecompPF/KppSolvePF against KppDecomp/KppSolve, bitwise, on a
+    sequence of matrices written into one reused buffer, every entry
+    overwritten before each factorisation as JacGhimj does in a persistent
+    lane's tile.  (They pass; the stall seen with them on the GPU is not an
+    arithmetic or ordering error of the routines that a host build shows.)"""
+    from kpp_amd import build
+    exe = build.OBJ / "kppgen"
+    if not exe.exists() or not (build.kpp_home() / "models" / f"{mech}.def").exists():
+        pytest.skip("kppgen or the KPP models not available")
+    gen = tmp_path / "gen"
+    gen.mkdir()
+    subprocess.run([str(exe), "--kpp-home", str(build.kpp_home()), "--root", mech, "-o", str(gen),
+                    "--pipelined", str(build.kpp_home() / "models" / f"{mech}.def")],
+                   check=True, capture_output=True)
+    (tmp_path / "hip").mkdir()
+    (tmp_path / "hip" / "hip_runtime.h").write_text(SHIM)
+    (tmp_path / "prog.cpp").write_text(PF_PROG % {"mech": mech})
+    so = tmp_path / "libpf.so"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-shared", "-fPIC", "-w",
+                    "-I", str(tmp_path), "-I", str(gen), str(tmp_path / "prog.cpp"), "-o", str(so)],
+                   check=True)
+    L = ctypes.CDLL(str(so))
+    rng = np.random.default_rng(17)
+    Gbuf = np.zeros(ref.sizes(mech)["LU_NONZERO"])
+    Xbuf = np.zeros(ref.sizes(mech)["NVAR"])
+    for _ in range(6):
+        G0, X0 = _matrix(mech, rng)
+        Gr, Xr = G0.copy(), X0.copy()
+        L.ref_lu(_p(Gr), _p(Xr))
+        Gbuf[:] = G0   # the same buffer every time, holding the previous factors before
+        Xbuf[:] = X0
+        L.pf_lu(_p(Gbuf), _p(Xbuf))
+        assert np.array_equal(Gbuf, Gr)
+        assert np.array_equal(Xbuf, Xr)
