@@ -2273,7 +2273,8 @@ std::vector<uint64_t> schedule(const std::vector<Chain>& ch, int dist, int& T) {
   return tab;
 }
 
-// Step descriptors, 16-bit fields, VALID bit 52.
+// Scheduling descriptors (a program before encode_sums/encode_ops): 16-bit
+// element indices, VALID bit 52.
 // Sum chains (Vdot, JVS): constant-pool index of the coefficient bits 0-15
 // (the coefficient itself is also tabled per step), source (A or B product)
 // bits 16-31, target bits 32-47, FIRST 48, LAST 49, ZERO 50 (the value is
@@ -2288,6 +2289,97 @@ uint64_t sum_desc(int coef, int src, int tgt, bool first, bool last, bool zero, 
 uint64_t op_desc(int a, int b, int t, bool first, bool last, bool div) {
   return (uint64_t)a | ((uint64_t)b << 16) | ((uint64_t)t << 32) | ((uint64_t)first << 48) |
          ((uint64_t)last << 49) | ((uint64_t)div << 50) | VALID;
+}
+
+// ---- the device encoding of a scheduled program ----
+// A cell's LDS (byte addresses; the kernel's dynamic LDS starts at 0):
+//   G region [0, GZ), GZ = 8*LU_NONZERO: the LU entries;
+//   T region from GZ: T-relative 0 is a junk slot (JUNK), 8 holds +0.0
+//   (ZERO), from 16 the products of Fun/Jac_SP (Fun's results after the A
+//   products), the solve's X then Z vectors, the error-norm terms.
+// Descriptor fields are 16-bit byte offsets from the region base the field
+// belongs to, so the device adds no base (it is the LDS instruction's
+// immediate offset).  Every step stores: a chain writes its running value to
+// its own target (a partial value is never read: a target's original value
+// is read only by the chain's own first step, other chains read it after the
+// chain's last step); an idle step leaves the value unchanged (acc - 0*0, or
+// acc + (-0.0)*0) and writes it back to the lane's previous target, or JUNK.
+constexpr uint32_t T_JUNK = 0, T_ZERO = 8, T_DATA = 16;
+
+// Sum programs: lo = product offset (T) | target offset << 16 (Fun: T, its
+// results region at fo; Jac_SP: G); hi bit 31 = FIRST.  Coefficient per
+// step: the chain's, 1.0 for the single step of an empty chain (0.0 * 1.0),
+// -0.0 for idle steps.
+void encode_sums(const std::vector<uint64_t>& tab, bool jvs, uint32_t gz, uint32_t fo,
+                 const std::vector<double>& pool, std::vector<uint64_t>& out,
+                 std::vector<double>& coef) {
+  const size_t steps = tab.size() / 64;
+  out.assign(tab.size(), 0);
+  coef.assign(tab.size(), 0.0);
+  const uint32_t junk = jvs ? gz + T_JUNK : T_JUNK;
+  for (int l = 0; l < 64; l++) {
+    uint32_t prev = junk;
+    for (size_t t = 0; t < steps; t++) {
+      const uint64_t d = tab[t * 64 + l];
+      uint32_t p = T_ZERO, s = prev, first = 0;
+      double c = -0.0;
+      if (d & VALID) {
+        const bool zero = (d >> 50) & 1;
+        const uint32_t src = (uint32_t)((d >> 16) & 0xFFFF), tgt = (uint32_t)((d >> 32) & 0xFFFF);
+        p = zero ? T_ZERO : T_DATA + 8 * src;
+        c = zero ? 1.0 : pool[d & 0xFFFF];
+        s = jvs ? 8 * tgt : fo + 8 * tgt;
+        first = (d >> 48) & 1;
+        prev = s;
+      }
+      out[t * 64 + l] = (uint64_t)p | ((uint64_t)s << 16) | ((uint64_t)first << 63);
+      coef[t * 64 + l] = c;
+    }
+  }
+}
+
+// LU (every field in G) and solve (a in G; b, target and init in T, X at
+// T_DATA, Z after it) programs: lo = a | b << 16; hi = target | init << 16,
+// target bit 0 = this lane divides (acc / [a]), bit 1 = some lane of the step
+// divides (the same in all lanes); init = the first step's initial value, or
+// JUNK when the step is not a chain's first (JUNK is the lowest address of
+// the T region: solve inits are above it, LU inits below it).
+void encode_ops(const std::vector<uint64_t>& tab, bool solve, uint32_t gz, int nvar,
+                std::vector<uint64_t>& out) {
+  const size_t steps = tab.size() / 64;
+  out.assign(tab.size(), 0);
+  const uint32_t junk = solve ? T_JUNK : gz + T_JUNK;
+  const uint32_t zero_a = gz + T_ZERO, zero_b = solve ? T_ZERO : gz + T_ZERO;
+  auto x_off = [&](uint32_t idx) { return solve ? T_DATA + 8 * idx : 8 * idx; };
+  for (int l = 0; l < 64; l++) {
+    uint32_t prev = junk;
+    for (size_t t = 0; t < steps; t++) {
+      const uint64_t d = tab[t * 64 + l];
+      uint32_t a = zero_a, b = zero_b, s = prev, init = junk, div = 0;
+      if (d & VALID) {
+        const uint32_t ai = (uint32_t)(d & 0xFFFF), bi = (uint32_t)((d >> 16) & 0xFFFF),
+                       ti = (uint32_t)((d >> 32) & 0xFFFF);
+        a = 8 * ai;
+        b = x_off(bi);
+        s = x_off(ti);
+        if ((d >> 48) & 1) {
+          // backward solve chains (targets in Z) start from the forward result
+          const uint32_t ii = (solve && ti >= (uint32_t)nvar) ? ti - (uint32_t)nvar : ti;
+          init = x_off(ii);
+        }
+        div = (d >> 50) & 1;
+        prev = s;
+      }
+      out[t * 64 + l] = (uint64_t)a | ((uint64_t)b << 16) | ((uint64_t)(s | div) << 32) |
+                        ((uint64_t)init << 48);
+    }
+  }
+  for (size_t t = 0; t < steps; t++) {
+    bool any = false;
+    for (int l = 0; l < 64; l++) any |= (out[t * 64 + l] >> 32) & 1;
+    if (any)
+      for (int l = 0; l < 64; l++) out[t * 64 + l] |= 2ull << 32;
+  }
 }
 
 void emit_table(std::ostream& o, const char* type, const std::string& name, size_t n,
@@ -2326,8 +2418,12 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
     maxf = std::max(maxf, (int)f.size());
   }
   const int na = (int)aprod.size(), nb = (int)bprod.size();
-  if (nnz >= 0xFFFF || 2 * nv >= 0xFFFF || na >= 0xFFFF || nb >= 0xFFFF)
-    why = "sizes beyond 16-bit indices";
+  // T region (doubles): ZERO, JUNK, then the largest of the Fun products and
+  // results, the Jac_SP products, the solve's X and Z
+  const int tsize = 2 + std::max(std::max(na + nv, nb), 2 * nv);
+  const uint32_t gz = 8u * (uint32_t)nnz, fo = T_DATA + 8u * (uint32_t)na;
+  if (gz + T_JUNK > 0xFFFF || 8u * (uint32_t)tsize > 0xFFFF)
+    why = "LDS byte offsets beyond 16 bits";
   auto pool_index = [&](double v) {
     size_t k = 0;
     while (k < pool.size() && !(pool[k] == v)) k++;
@@ -2469,12 +2565,17 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
       return std::to_string(v) + "u";
     };
   };
-  auto coef_of = [&](const std::vector<uint64_t>& tab) {
-    return [&tab, &pool](size_t i) {
-      char b[40];
-      snprintf(b, sizeof b, "%.17g", (tab[i] && !pool.empty()) ? pool[tab[i] & 0xFFFF] : 0.0);
-      return std::string(b);
-    };
+  std::vector<uint64_t> efun, ejac, elu, esol;
+  std::vector<double> cfun, cjac;
+  encode_sums(tfun, false, gz, fo, pool, efun, cfun);
+  encode_sums(tjac, true, gz, fo, pool, ejac, cjac);
+  encode_ops(tlu, false, gz, nv, elu);
+  encode_ops(tsol, true, gz, nv, esol);
+  auto dbl = [](double v) {
+    if (v == 0.0) return std::string(std::signbit(v) ? "-0.0" : "0.0");
+    char b[40];
+    snprintf(b, sizeof b, "%.17g", v);
+    return std::string(b);
   };
   // the tables live in a namespace (device variables cannot be static data
   // members); the struct hands out their addresses
@@ -2487,12 +2588,12 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
     snprintf(b, sizeof b, "%.17g", pool[i]);
     return std::string(b);
   });
-  emit_table(o, "unsigned long long", "FUN", tfun.size(), [&](size_t i) { return hex(tfun[i]); });
-  emit_table(o, "unsigned long long", "JAC", tjac.size(), [&](size_t i) { return hex(tjac[i]); });
-  emit_table(o, "double", "FUN_COEF", tfun.size(), coef_of(tfun));
-  emit_table(o, "double", "JAC_COEF", tjac.size(), coef_of(tjac));
-  emit_table(o, "unsigned long long", "LU", tlu.size(), [&](size_t i) { return hex(tlu[i]); });
-  emit_table(o, "unsigned long long", "SOL", tsol.size(), [&](size_t i) { return hex(tsol[i]); });
+  emit_table(o, "unsigned long long", "FUN", efun.size(), [&](size_t i) { return hex(efun[i]); });
+  emit_table(o, "unsigned long long", "JAC", ejac.size(), [&](size_t i) { return hex(ejac[i]); });
+  emit_table(o, "double", "FUN_COEF", cfun.size(), [&](size_t i) { return dbl(cfun[i]); });
+  emit_table(o, "double", "JAC_COEF", cjac.size(), [&](size_t i) { return dbl(cjac[i]); });
+  emit_table(o, "unsigned long long", "LU", elu.size(), [&](size_t i) { return hex(elu[i]); });
+  emit_table(o, "unsigned long long", "SOL", esol.size(), [&](size_t i) { return hex(esol[i]); });
   emit_table(o, "unsigned", "DIAG", nv, [&](size_t i) { return std::to_string(S.lu_diag[i]) + "u"; });
   o << "}  // namespace " << ns << "_coop_data\n";
   o << "struct " << ns << "_coop {\n";
@@ -2500,6 +2601,10 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   o << "static constexpr int NA = " << na << ", NB = " << nb << ", MAXF = " << maxf
     << ", NCONST = " << pool.size() << ", NA_PAD = " << na_pad << ", NB_PAD = " << nb_pad
     << ", PAD = " << PAD << ";\n";
+  o << "// LDS byte offsets: G region size, Fun results (T-relative), T region doubles\n";
+  o << "static constexpr unsigned GZ = " << gz << "u, FO = " << fo << "u, JSLOT = " << T_JUNK
+    << "u, ZSLOT = " << T_ZERO << "u;\n";
+  o << "static constexpr int TSIZE = " << tsize << ";\n";
   o << "static constexpr int FUN_STEPS = " << tf << ", JAC_STEPS = " << tj
     << ", LU_STEPS = " << tl << ", SOL_STEPS = " << ts << ";\n";
   const char* names[] = {"APROD", "BPROD", "CONSTS", "FUN", "JAC", "LU", "SOL", "DIAG",

@@ -23,11 +23,20 @@
 // kernels' (and the CPU oracle's) bit for bit.
 //
 // Program execution: a step reads its operands from LDS, updates the lane's
-// running value and, at the chain's end, stores it.  The loads of step t+1
-// are issued before the store of step t (the schedule puts every read of a
-// chain's result at least two steps after its last step), and LDS
-// operations of one wavefront execute in order, so no barrier is needed
-// inside a program.
+// running value and stores it to the chain's target (every step stores: a
+// partial value is never read, since a target's original value is read only
+// by its own chain's first step and other chains read it after the chain's
+// last step; an idle step leaves the value unchanged and writes it back to
+// the lane's previous target).  The loads of step t+1 are issued before the
+// store of step t (the schedule puts every read of a chain's result at least
+// two steps after its last step), and LDS operations of one wavefront
+// execute in order, so no barrier is needed inside a program.
+//
+// Descriptor fields are LDS byte offsets from the base of the region they
+// address (G at LDS address 0; T at C::GZ, a compile-time constant that the
+// LDS instructions take as their immediate offset), so a step spends one
+// integer operation per operand; the dynamic LDS of the kernel starts at 0
+// because the kernel declares no static LDS.
 #pragma once
 #include "ros_device.h"
 
@@ -49,12 +58,15 @@ struct CoopLayout {
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
-// LDS of one cell: G[LU_NONZERO], then T (products, solve vectors X|Z,
-// error terms).
+// LDS of one cell: G[LU_NONZERO] at 0, then the T region at C::GZ: a junk
+// slot (C::JSLOT, written by idle steps), +0.0 (C::ZSLOT, read by idle
+// steps), then the products, Fun's results, the solve vectors X|Z and the
+// error terms.
 template <class M, class C>
 struct CoopLds {
-  static constexpr int TSIZE = cmax(cmax(C::NA, C::NB), 2 * M::NVAR);
-  static constexpr size_t BYTES = sizeof(double) * (size_t)(M::LU_NONZERO + TSIZE);
+  static_assert(C::GZ == 8u * M::LU_NONZERO && C::JSLOT == 0u && C::ZSLOT == 8u,
+                "LDS layout of the cooperative programs");
+  static constexpr size_t BYTES = sizeof(double) * (size_t)(M::LU_NONZERO + C::TSIZE);
 };
 
 // Global-memory accessor for the emitted rate routines (stride 1).
@@ -125,82 +137,113 @@ __device__ __noinline__ void coop_products(const unsigned* Pg, int N, gcdouble* 
   }
 }
 
-// A sum program (Vdot or JVS chains over the products in T).  At a chain's
-// end the value goes to out[t] (JVS: G[t] = -value, +ghinv on the diagonal).
-// Idle steps (descriptor 0) read T[0] and store nothing.
-template <bool JVS, class Out>
+// LDS at a byte address (the kernel's dynamic LDS starts at address 0)
+__device__ __forceinline__ double lds_ld(unsigned addr) { return *(ldouble*)(size_t)addr; }
+__device__ __forceinline__ void lds_st(unsigned addr, double v) { *(ldouble*)(size_t)addr = v; }
+
+// A sum program (Vdot or JVS chains over the products in T).  Descriptor:
+// lo = product offset (T) | target offset << 16 (Fun: T, Jac_SP: G), hi bit
+// 31 = the chain's first step; per-step coefficient table.  Step:
+// term = coef * T[p]; acc = first ? term : acc + term; the target gets acc
+// (Jac_SP: -acc, the Ghimj build; the diagonal's +1/(h gamma) is added
+// afterwards).  Products are not written here, so their reads run two steps
+// ahead of use.
+template <bool JVS, unsigned GZ>
 __device__ __noinline__ void coop_sums(const unsigned long long* progg, const double* coefg,
-                                       int steps, ldouble* T, Out* out, double ghinv, int lane) {
+                                       int steps, int lane) {
   gcu64* prog = (gcu64*)progg;
   gcdouble* cf = (gcdouble*)coefg;
+  constexpr unsigned SB = JVS ? 0u : GZ;  // base of the target field
   unsigned long long r[kRing];
   double c[kRing];
+  gcu64* q = prog + lane;
+  gcdouble* qc = cf + lane;
 #pragma unroll
   for (int k = 0; k < kRing; k++) {
-    r[k] = prog[k * TILE + lane];
-    c[k] = cf[k * TILE + lane];
+    r[k] = q[k * TILE];
+    c[k] = qc[k * TILE];
   }
+  q += kRing * TILE;
+  qc += kRing * TILE;
+  double x0 = lds_ld(GZ + ((unsigned)r[0] & 0xFFFFu)), x1 = lds_ld(GZ + ((unsigned)r[1] & 0xFFFFu));
   double acc = 0.0;
   for (int base = 0; base < steps; base += kRing) {
 #pragma unroll
     for (int k = 0; k < kRing; k++) {
       const unsigned long long d = r[k];
-      const double coef = c[k];
-      const int tn = (base + k + kRing) * TILE + lane;
-      r[k] = prog[tn];
-      c[k] = cf[tn];
-      const double x = T[(d >> 16) & 0xFFFF];
-      const double pr = coef * x;
-      const double term = ((d >> 50) & 1) ? 0.0 : pr;
+      // product of step base+k+2 (slot k+2; slots 0 and 1 hold the next
+      // round's first two steps, refilled at k = 0 and 1)
+      const double x2 = lds_ld(GZ + ((unsigned)r[(k + 2) % kRing] & 0xFFFFu));
+      const double term = c[k] * x0;
       const double sum = acc + term;
-      acc = ((d >> 48) & 1) ? term : sum;
-      if ((d >> 49) & 1) {
-        double v = acc;
-        if (JVS) {
-          v = -(acc);
-          if ((d >> 51) & 1) v = v + ghinv;
-        }
-        out[(d >> 32) & 0xFFFF] = v;
-      }
+      acc = ((int)(unsigned)(d >> 32) < 0) ? term : sum;
+      lds_st(SB + ((unsigned)d >> 16), JVS ? -acc : acc);
+      x0 = x1;
+      x1 = x2;
+      // refilled after the last use (see coop_program)
+      __builtin_amdgcn_sched_barrier(0);
+      r[k] = q[k * TILE];
+      c[k] = qc[k * TILE];
     }
+    q += kRing * TILE;
+    qc += kRing * TILE;
   }
 }
 
-// An elimination program over LDS: LU (X = G) or the triangular solves
-// (X = T, forward results in X[0:NVAR), backward in X[NVAR:2NVAR)).
-// Step: FIRST: acc = X[init(t)]; DIV: acc = acc / G[a], else
-// acc = acc - G[a]*X[b]; LAST: X[t] = acc.  The LDS operands of step t+1 are
-// read (unconditionally; idle steps read entry 0) before step t stores.
-template <int NVAR, bool SOLVE>
-__device__ __noinline__ void coop_program(const unsigned long long* progg, int steps, ldouble* G,
-                                          ldouble* X, int lane) {
+// An elimination program over LDS: LU (every field in G) or the triangular
+// solves (a in G; b, target and init in T, forward results in X = T[0:NVAR),
+// backward in Z = T[NVAR:2NVAR)).  Descriptor: lo = a | b << 16; hi =
+// target | init << 16; target bit 0: this lane divides, bit 1: some lane of
+// the step does (the same in every lane); init = JUNK (the first address of
+// T) unless the step is a chain's first: a solve's inits lie above JUNK, the
+// LU's (in G) below it, so one comparison of the whole word tells.  Step: v = first ? X[init] : acc; v = divides ?
+// v / G[a] : v - G[a]*X[b]; X[target] = v.  The LDS operands of step t+1 are
+// read before step t stores.
+template <bool SOLVE, unsigned GZ>
+__device__ __noinline__ void coop_program(const unsigned long long* progg, int steps, int lane) {
   gcu64* prog = (gcu64*)progg;
-  auto init_of = [](unsigned t) {
-    // backward solve chains start from the forward result X[i] (t = NVAR+i)
-    return (SOLVE && t >= (unsigned)NVAR) ? t - (unsigned)NVAR : t;
-  };
+  constexpr unsigned XB = SOLVE ? GZ : 0u;  // base of b, target, init
   unsigned long long r[kRing];
+  gcu64* q = prog + lane;
 #pragma unroll
-  for (int k = 0; k < kRing; k++) r[k] = prog[k * TILE + lane];
-  double ga = G[r[0] & 0xFFFF], xb = X[(r[0] >> 16) & 0xFFFF],
-         xt = X[init_of((r[0] >> 32) & 0xFFFF)], acc = 0.0;
+  for (int k = 0; k < kRing; k++) r[k] = q[k * TILE];
+  q += kRing * TILE;
+  auto operands = [](unsigned long long d, double& ga, double& xb, double& xi) {
+    const unsigned lo = (unsigned)d, hi = (unsigned)(d >> 32);
+    ga = lds_ld(lo & 0xFFFFu);
+    xb = lds_ld(XB + (lo >> 16));
+    xi = lds_ld(XB + (hi >> 16));
+  };
+  double ga, xb, xi, acc = 0.0;
+  operands(r[0], ga, xb, xi);
   for (int base = 0; base < steps; base += kRing) {
 #pragma unroll
     for (int k = 0; k < kRing; k++) {
-      const unsigned long long d = r[k];
-      r[k] = prog[(base + k + kRing) * TILE + lane];
+      const unsigned hi = (unsigned)(r[k] >> 32);
       // step base+k+1: slot k+1, or (k = RING-1) slot 0, refilled at k = 0
-      const unsigned long long dn = r[(k + 1) % kRing];
-      const double gan = G[dn & 0xFFFF], xbn = X[(dn >> 16) & 0xFFFF],
-                   xtn = X[init_of((dn >> 32) & 0xFFFF)];
-      if ((d >> 48) & 1) acc = xt;
-      if ((d >> 50) & 1) acc = acc / ga;
-      else acc = acc - ga * xb;
-      if ((d >> 49) & 1) X[(d >> 32) & 0xFFFF] = acc;
+      double gan, xbn, xin;
+      operands(r[(k + 1) % kRing], gan, xbn, xin);
+      const bool first = SOLVE ? (hi >= 0x10000u) : (hi < (GZ << 16));
+      const double v0 = first ? xi : acc;
+      double v = v0 - ga * xb;
+      if (__builtin_amdgcn_readfirstlane(hi) & 2u) {
+        // a step with divisions (the barrier keeps the division out of the
+        // other steps)
+        __builtin_amdgcn_sched_barrier(0);
+        if (hi & 1u) v = v0 / ga;
+      }
+      lds_st(XB + (hi & 0xFFF8u), v);
+      acc = v;
       ga = gan;
       xb = xbn;
-      xt = xtn;
+      xi = xin;
+      // the slot is refilled after its last use, so the load can land in
+      // the same registers (no copies, and no wait for the ring, at the
+      // round's end)
+      __builtin_amdgcn_sched_barrier(0);
+      r[k] = q[k * TILE];
     }
+    q += kRing * TILE;
   }
 }
 
@@ -239,29 +282,36 @@ struct CoopCell {
   // every lane sees the others' stores (global memory and LDS)
   __device__ __forceinline__ static void sync() { __syncthreads(); }
 
-  // Fun(V) -> out (a global row): products, then the Vdot chains
+  // Fun(V) -> out (a global row): products, then the Vdot chains into the
+  // results region of T, copied out
   __device__ __forceinline__ void fun(gcdouble* V, gdouble* out) const {
     coop_products<C::MAXF, C::NA_PAD>(C::APROD(), C::NA, V, (gcdouble*)row(L::RCT),
                                       (gcdouble*)row(L::FIX), cst(), T, lane);
     sync();
-    coop_sums<false>(C::FUN(), C::FUN_COEF(), C::FUN_STEPS, T, out, 0.0, lane);
+    coop_sums<false, C::GZ>(C::FUN(), C::FUN_COEF(), C::FUN_STEPS, lane);
+    sync();
+    vec([&](int i) { out[i] = lds_ld(C::GZ + C::FO + 8u * (unsigned)i); });
     sync();
   }
 
   // Jac_SP at (Y, rates) fused with the Ghimj build into G (LDS): G = -JVS,
-  // G[diag] += ghinv.  Returns the first row (1-based) whose diagonal is
+  // then G[diag] += ghinv.  Returns the first row (1-based) whose diagonal is
   // exactly zero (the KppDecomp test of util/sutil.c:8), or 0.
   __device__ __forceinline__ int jac_ghimj(double ghinv) const {
     coop_products<C::MAXF, C::NB_PAD>(C::BPROD(), C::NB, (gcdouble*)row(L::Y),
                                       (gcdouble*)row(L::RCT), (gcdouble*)row(L::FIX), cst(), T,
                                       lane);
     sync();
-    coop_sums<true>(C::JAC(), C::JAC_COEF(), C::JAC_STEPS, T, G, ghinv, lane);
+    coop_sums<true, C::GZ>(C::JAC(), C::JAC_COEF(), C::JAC_STEPS, lane);
     sync();
     int ising = 0x7FFFFFFF;
     gcu32* dg = (gcu32*)C::DIAG();
-    for (int i = lane; i < M::NVAR; i += TILE)
-      if (G[dg[i]] == 0.0 && i + 1 < ising) ising = i + 1;
+    for (int i = lane; i < M::NVAR; i += TILE) {
+      const double gd = G[dg[i]] + ghinv;
+      G[dg[i]] = gd;
+      if (gd == 0.0 && i + 1 < ising) ising = i + 1;
+    }
+    sync();
     for (int off = 32; off > 0; off >>= 1) {
       const int o = __shfl_xor(ising, off);
       ising = o < ising ? o : ising;
@@ -270,12 +320,12 @@ struct CoopCell {
   }
 
   __device__ __forceinline__ void decomp() const {
-    coop_program<M::NVAR, false>(C::LU(), C::LU_STEPS, G, G, lane);
+    coop_program<false, C::GZ>(C::LU(), C::LU_STEPS, lane);
     sync();
   }
   // solve G x = T[0:NVAR) in place; the result is left in T[NVAR:2NVAR)
   __device__ __forceinline__ void solve() const {
-    coop_program<M::NVAR, true>(C::SOL(), C::SOL_STEPS, G, T, lane);
+    coop_program<true, C::GZ>(C::SOL(), C::SOL_STEPS, lane);
     sync();
   }
 
@@ -628,8 +678,14 @@ void ros_coop_kernel(RosParams p) {
   using L = CoopLayout<M, S>;
   extern __shared__ double coop_lds[];
   const int lane = threadIdx.x;
+  // the programs address LDS absolutely: G at 0, the T region at C::GZ (the
+  // LDS-space address of the dynamic array, not its generic one)
+  if ((unsigned)(size_t)(ldouble*)coop_lds != 0u) return;
   CoopCell<M, C, S> c{(gdouble*)(p.scratch + (size_t)blockIdx.x * L::ROWS), (ldouble*)coop_lds,
-                      (ldouble*)coop_lds + M::LU_NONZERO, lane};
+                      (ldouble*)coop_lds + M::LU_NONZERO + 2, lane};
+  // the T region's ZERO slot holds +0.0 for the whole launch (idle steps read it)
+  if (lane == 0) lds_st(C::GZ + C::ZSLOT, 0.0);
+  __syncthreads();
   for (;;) {
     int k = 0;
     if (lane == 0) k = atomicAdd(p.queue, 1);

@@ -1,9 +1,13 @@
 """The mechanism compiler's cooperative programs (kppgen --coop), run on the
-host by a lockstep emulation of the device interpreter (kpp_amd/csrc/
-ros_coop.h): 64 lanes, and at every step all lanes first read the operands
-of the next step, then compute the current step and store.  A schedule that
-read a value in the step after it is stored, or a table entry naming a
-wrong operand, changes the results.
+host by a lockstep emulation of the device interpreters (kpp_amd/csrc/
+ros_coop.h) over the cell's LDS image (byte addresses, G at 0, the T region
+at GZ): 64 lanes; the sum programs read each step's product two steps
+ahead, the elimination programs read the operands of step t+1 before step t
+stores; every step stores, idle steps included, and the device's whole
+rounds of 8 steps run into the idle padding.  The LDS starts as NaN except
+the +0.0 slot, which must stay +0.0.  A schedule that read a value before it
+is final, an idle step that changed a value, or a table entry naming a wrong
+operand changes the results.
 
 Bar: bitwise equality with the generated straight-line routines (Fun,
 JacGhimj, KppDecomp, KppSolve) of the same header, which the oracle's
@@ -32,9 +36,11 @@ using std::acos; using std::fabs;
 """
 
 PROG = r"""#include "%(mech)s_hip.h"
+#include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 struct HA {
   using value_type = double;
   double* b;
@@ -42,7 +48,6 @@ struct HA {
 };
 using M = kppm_%(mech)s;
 using C = kppm_%(mech)s_coop;
-static const unsigned long long IDLE = 0;  // idle (lane, step); real steps have bit 52
 extern "C" int enabled() { return C::ENABLED ? 1 : 0; }
 
 // ---- reference: the straight-line routines ----
@@ -58,6 +63,23 @@ extern "C" void ref_lu(double* G, double* X) {
 }
 
 #if %(enabled)d
+// the cell's LDS as the device sees it: byte addresses, G at 0, T at GZ
+static std::vector<double> lds;
+static double& at(unsigned addr) {
+  if (addr %% 8 || addr / 8 >= lds.size()) abort();
+  return lds[addr / 8];
+}
+static constexpr int RING = 8;
+static int rounds(int steps) { return (steps + RING - 1) / RING * RING; }
+static void fresh() {
+  // everything NaN (a read of memory no step should read shows in the
+  // results) except the ZERO slot
+  lds.assign(M::LU_NONZERO + C::TSIZE, std::nan(""));
+  at(C::GZ + C::ZSLOT) = 0.0;
+}
+static void zero_slot_intact() {
+  if (!(at(C::GZ + C::ZSLOT) == 0.0) || std::signbit(at(C::GZ + C::ZSLOT))) abort();
+}
 static double factor(unsigned code, const double* V, const double* F, const double* R) {
   const unsigned idx = code & 0x3FFFFFFFu;
   switch (code >> 30) {
@@ -68,7 +90,7 @@ static double factor(unsigned code, const double* V, const double* F, const doub
   }
 }
 static void products(const unsigned* P, int N, int NPAD, const double* V, const double* F,
-                     const double* R, double* T) {
+                     const double* R) {
   for (int q = 0; q < N; q++) {
     double p = factor(P[q], V, F, R);
     for (int f = 1; f < C::MAXF; f++) {
@@ -76,83 +98,87 @@ static void products(const unsigned* P, int N, int NPAD, const double* V, const 
       if (code == 0xFFFFFFFFu) break;
       p = p * factor(code, V, F, R);
     }
-    T[q] = p;
+    at(C::GZ + 16 + 8 * q) = p;
   }
 }
-// sum chains in lockstep; the coefficient comes from the per-step table
-// (which must agree with the constant pool index of the descriptor)
-template <class Store>
-static void sums(const unsigned long long* prog, const double* coef, int steps, const double* T,
-                 Store store) {
-  double acc[64] = {0};
-  for (int t = 0; t < steps; t++)
+// a sum program in lockstep, as the device runs it: whole rounds of RING
+// steps, each step's product read two steps ahead (before the stores of the
+// two steps in between), every step storing
+static void sums(const unsigned long long* prog, const double* coef, int steps, bool jvs) {
+  const unsigned sb = jvs ? 0u : C::GZ;
+  const int n = rounds(steps);
+  if (n + 2 > steps + C::PAD) abort();
+  double acc[64] = {0}, x0[64], x1[64], x2[64];
+  auto prod = [&](int t, int l) { return at(C::GZ + ((unsigned)prog[t * 64 + l] & 0xFFFFu)); };
+  for (int l = 0; l < 64; l++) { x0[l] = prod(0, l); x1[l] = prod(1, l); }
+  for (int t = 0; t < n; t++) {
+    for (int l = 0; l < 64; l++) x2[l] = prod(t + 2, l);
     for (int l = 0; l < 64; l++) {
       const unsigned long long d = prog[t * 64 + l];
-      if (d == IDLE) continue;
-      if (!((d >> 52) & 1)) abort();
-      const double c = coef[t * 64 + l];
-      if (!(c == C::CONSTS()[d & 0xFFFF]) && !((d >> 50) & 1)) abort();
-      const double term = ((d >> 50) & 1) ? 0.0 : c * T[(d >> 16) & 0xFFFF];
-      acc[l] = ((d >> 48) & 1) ? term : acc[l] + term;
-      if ((d >> 49) & 1) store(d, acc[l]);
-    }
-  // the padding past the last step is idle
-  for (int t = steps; t < steps + C::PAD; t++)
-    for (int l = 0; l < 64; l++)
-      if (prog[t * 64 + l] != IDLE) abort();
-}
-extern "C" void coop_fun(double* V, double* F, double* R, double* out) {
-  static double T[C::NA > 0 ? C::NA : 1];
-  products(C::APROD(), C::NA, C::NA_PAD, V, F, R, T);
-  sums(C::FUN(), C::FUN_COEF(), C::FUN_STEPS, T,
-       [&](unsigned long long d, double a) { out[(d >> 32) & 0xFFFF] = a; });
-}
-extern "C" void coop_jac(double* V, double* F, double* R, double ghinv, double* G) {
-  static double T[C::NB > 0 ? C::NB : 1];
-  products(C::BPROD(), C::NB, C::NB_PAD, V, F, R, T);
-  sums(C::JAC(), C::JAC_COEF(), C::JAC_STEPS, T, [&](unsigned long long d, double a) {
-    double g = -(a);
-    if ((d >> 51) & 1) g = g + ghinv;
-    G[(d >> 32) & 0xFFFF] = g;
-  });
-}
-// elimination program in lockstep: at step t every lane reads the operands
-// of step t+1, then computes step t and stores
-static void program(const unsigned long long* prog, int steps, double* G, double* X, bool solve,
-                    int nvar) {
-  auto init_of = [&](unsigned t) { return (solve && t >= (unsigned)nvar) ? t - nvar : t; };
-  double acc[64] = {0}, ga[64] = {0}, xb[64] = {0}, xt[64] = {0};
-  unsigned long long d[64];
-  // every lane reads its three operands each step, idle ones entry 0
-  auto load = [&](int t, int l, double& a, double& b, double& c) {
-    const unsigned long long e = prog[t * 64 + l];
-    a = G[e & 0xFFFF];
-    b = X[(e >> 16) & 0xFFFF];
-    c = X[init_of((e >> 32) & 0xFFFF)];
-    return e;
-  };
-  for (int l = 0; l < 64; l++) d[l] = load(0, l, ga[l], xb[l], xt[l]);
-  for (int t = 0; t < steps; t++) {
-    unsigned long long dn[64];
-    double gan[64] = {0}, xbn[64] = {0}, xtn[64] = {0};
-    for (int l = 0; l < 64; l++) dn[l] = load(t + 1, l, gan[l], xbn[l], xtn[l]);
-    for (int l = 0; l < 64; l++) {
-      if (d[l] != IDLE && !((d[l] >> 52) & 1)) abort();
-      if ((d[l] >> 48) & 1) acc[l] = xt[l];
-      acc[l] = ((d[l] >> 50) & 1) ? acc[l] / ga[l] : acc[l] - ga[l] * xb[l];
-      if ((d[l] >> 49) & 1) X[(d[l] >> 32) & 0xFFFF] = acc[l];
-    }
-    for (int l = 0; l < 64; l++) {
-      d[l] = dn[l]; ga[l] = gan[l]; xb[l] = xbn[l]; xt[l] = xtn[l];
+      const double term = coef[t * 64 + l] * x0[l];
+      const double sum = acc[l] + term;
+      acc[l] = ((int)(unsigned)(d >> 32) < 0) ? term : sum;
+      at(sb + ((unsigned)d >> 16)) = jvs ? -acc[l] : acc[l];
+      x0[l] = x1[l];
+      x1[l] = x2[l];
     }
   }
+  zero_slot_intact();
+}
+extern "C" void coop_fun(double* V, double* F, double* R, double* out) {
+  fresh();
+  products(C::APROD(), C::NA, C::NA_PAD, V, F, R);
+  sums(C::FUN(), C::FUN_COEF(), C::FUN_STEPS, false);
+  for (int i = 0; i < M::NVAR; i++) out[i] = at(C::GZ + C::FO + 8 * i);
+}
+extern "C" void coop_jac(double* V, double* F, double* R, double ghinv, double* G) {
+  fresh();
+  products(C::BPROD(), C::NB, C::NB_PAD, V, F, R);
+  sums(C::JAC(), C::JAC_COEF(), C::JAC_STEPS, true);
+  for (int i = 0; i < M::NVAR; i++) at(8 * C::DIAG()[i]) = at(8 * C::DIAG()[i]) + ghinv;
+  for (int n = 0; n < M::LU_NONZERO; n++) G[n] = at(8 * n);
+}
+// an elimination program in lockstep: at step t every lane reads the
+// operands of step t+1, then computes step t and stores
+static void program(const unsigned long long* prog, int steps, bool solve) {
+  const unsigned xb = solve ? C::GZ : 0u, noinit = solve ? C::JSLOT : C::GZ + C::JSLOT;
+  const int n = rounds(steps);
+  if (n + 1 > steps + C::PAD) abort();
+  double acc[64] = {0}, ga[64], xbv[64], xi[64], gn[64], bn[64], in[64];
+  auto load = [&](int t, int l, double& a, double& b, double& c) {
+    const unsigned long long e = prog[t * 64 + l];
+    const unsigned lo = (unsigned)e, hi = (unsigned)(e >> 32);
+    a = at(lo & 0xFFFFu);
+    b = at(xb + (lo >> 16));
+    c = at(xb + (hi >> 16));
+  };
+  for (int l = 0; l < 64; l++) load(0, l, ga[l], xbv[l], xi[l]);
+  for (int t = 0; t < n; t++) {
+    for (int l = 0; l < 64; l++) load(t + 1, l, gn[l], bn[l], in[l]);
+    const unsigned step_div = (unsigned)(prog[t * 64] >> 32) & 2u;
+    for (int l = 0; l < 64; l++) {
+      const unsigned hi = (unsigned)(prog[t * 64 + l] >> 32);
+      if (((hi & 2u) != step_div) || ((hi & 1u) && !step_div)) abort();
+      // the device's test of a chain's first step, and what it stands for
+      const bool first = solve ? (hi >= 0x10000u) : (hi < (C::GZ << 16));
+      if (first != ((hi >> 16) != noinit)) abort();
+      const double v0 = first ? xi[l] : acc[l];
+      const double v = (step_div && (hi & 1u)) ? v0 / ga[l] : v0 - ga[l] * xbv[l];
+      at(xb + (hi & 0xFFF8u)) = v;
+      acc[l] = v;
+    }
+    for (int l = 0; l < 64; l++) { ga[l] = gn[l]; xbv[l] = bn[l]; xi[l] = in[l]; }
+  }
+  zero_slot_intact();
 }
 extern "C" void coop_lu(double* G, double* X) {
-  static double T[2 * M::NVAR];
-  program(C::LU(), C::LU_STEPS, G, G, false, M::NVAR);
-  for (int i = 0; i < M::NVAR; i++) T[i] = X[i];
-  program(C::SOL(), C::SOL_STEPS, G, T, true, M::NVAR);
-  for (int i = 0; i < M::NVAR; i++) X[i] = T[M::NVAR + i];
+  fresh();
+  for (int n = 0; n < M::LU_NONZERO; n++) at(8 * n) = G[n];
+  program(C::LU(), C::LU_STEPS, false);
+  for (int n = 0; n < M::LU_NONZERO; n++) G[n] = at(8 * n);
+  for (int i = 0; i < M::NVAR; i++) at(C::GZ + 16 + 8 * i) = X[i];
+  program(C::SOL(), C::SOL_STEPS, true);
+  for (int i = 0; i < M::NVAR; i++) X[i] = at(C::GZ + 16 + 8 * (M::NVAR + i));
 }
 #else
 extern "C" void coop_fun(double*, double*, double*, double*) {}
@@ -161,7 +187,7 @@ extern "C" void coop_lu(double*, double*) {}
 #endif
 """
 
-MECHS = ["small_strato", "saprc99", "saprcnov", "mcm"]
+MECHS = ["small_strato", "saprc99", "saprcnov", "linear6", "mcm"]
 
 
 def _p(a):
