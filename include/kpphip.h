@@ -100,7 +100,8 @@ typedef struct {
   const double* rconst;  /* device [NREACT][ld] rate constants supplied by the caller or NULL */
   int* ierr;             /* device [ncells] Rosenbrock return code per cell */
   int* istat;            /* device [8][ld] IPAR[10..17] per cell, or NULL */
-  double* rstat;         /* device [2][ld] Texit, Hexit per cell, or NULL */
+  double* rstat;         /* device [2][ld] Texit, Hexit per cell, or NULL
+                            (kpp_integrate_f90_dev, kpp_sdirk_dev: [3][ld]) */
   const int* order;      /* device [ncells] processing order (a permutation) or NULL */
   double* hfirst;        /* device [ncells] step log: size of each cell's first accepted step
                             (0 if none was accepted), or NULL.  Not a reference output. */
@@ -158,6 +159,169 @@ int kpp_rosenbrock_dev(const char* mech, const kpp_batch* b, const int IPAR[20],
 int kpp_rosenbrock_dev_static(const char* mech, const kpp_batch* b, const int IPAR[20],
                               const double RPAR[20], const double* atol, const double* rtol,
                               const kpp_launch* l, void* stream);
+
+/* ---- the F90 integrator (int/rosenbrock.f90) ----
+ * KPP's Fortran 90 Rosenbrock is a separate implementation with its own
+ * controls and outputs.  kpp_integrate_f90_dev replaces its
+ *   SUBROUTINE INTEGRATE(TIN, TOUT, ICNTRL_U, RCNTRL_U, ISTATUS_U, RSTATUS_U,
+ *                        IERR_U)                   int/rosenbrock.f90:43-125
+ * for every cell of a device batch, with the semantics of its Rosenbrock
+ * (:128-446) and ros_Integrator (:492-686):
+ *  - ICNTRL_U/RCNTRL_U are merged as INTEGRATE does (ICNTRL = 0 with
+ *    ICNTRL(15) = 5, then ICNTRL_U where /= 0; RCNTRL_U where > 0); either
+ *    may be NULL (an absent optional argument);
+ *  - ICNTRL(1) autonomous, (2) 0 = vector / 1 = scalar tolerances, (3) method
+ *    0,4 Rodas3, 1 Ros2, 2 Ros3, 3 Ros4, 5 Rodas4, 6 Rang3, 7 Rodas3.1,
+ *    (4) max steps (0 = 200000), (15) rate updates (util/util.f90:133-159:
+ *    bit 1 Update_RCONST at every Fun/Jac, with bit 4 SUN from Update_SUN of
+ *    the time, else the cell's b->sun; without bit 1 the rates stay at their
+ *    values at TIN), (16) = 1 clips accepted states at zero;
+ *  - RCNTRL(1:7) Hmin, Hmax, Hstart (0: MAX(Hmin, 1e-5)), FacMin, FacMax,
+ *    FacRej, FacSafe;
+ *  - b->tstart/tend (or tstart0/tend0) are TIN/TOUT; b->sun_mode is ignored;
+ *    b->hstart per cell overrides RCNTRL(3); b->rconst supplies the rates;
+ *  - b->istat [8][ld] is ISTATUS(1:8); b->rstat must hold [3][ld]:
+ *    RSTATUS(1:3) = Texit, Hexit, Hnew of the last accepted step (0 if none);
+ *  - passive: host [NVAR] or NULL; passive[i] != 0 leaves species i out of
+ *    the error norm (NonPassiveSpc, src/gen.c:3355-3384: ATOL(i) >=
+ *    PassiveSpc_ATOL_Threshold);
+ *  - plain, persistent and register kernels; FP64 only (lu_fp32 and
+ *    KPP_VARIANT_COOP are KPP_E_ARG).
+ * Per-cell codes 1, -1..-8 as int/rosenbrock.f90:452-489; -9 also for a NaN
+ * error estimate (gfortran's MAX(Err, 1e-10) would accept it).  Returns 0, a
+ * parameter error -1..-5 (every cell's ierr set to it), or KPP_E_*. */
+int kpp_integrate_f90_dev(const char* mech, const kpp_batch* b, const int ICNTRL_U[20],
+                          const double RCNTRL_U[20], const double* atol, const double* rtol,
+                          const int* passive, const kpp_launch* l, void* workspace,
+                          size_t workspace_bytes, void* stream);
+/* The INTEGRATE of int/rosenbrock_h211b_qssa.f90 (:56-138): the F90
+ * integrator above with its step-size control replaced, when RCNTRL(15) = bb
+ * and RCNTRL(16) = kk are both set, by the H211b digital filter (:882-892;
+ * Soderlind, ACM TOMS 29 (2003)): Fac = (1/Err)^(1/(bb kk)) *
+ * (1/Err_prev)^(1/(bb kk)) * Fac_prev^(-1/bb), without the FacMin/FacMax
+ * bounds, the filter's memory starting at 1; the error norm over all
+ * species.  The QSSA options (ICNTRL(18) /= 0) are not built: KPP_E_ARG. */
+int kpp_integrate_h211b_dev(const char* mech, const kpp_batch* b, const int ICNTRL_U[20],
+                            const double RCNTRL_U[20], const double* atol, const double* rtol,
+                            const kpp_launch* l, void* workspace, size_t workspace_bytes,
+                            void* stream);
+/* workspace of kpp_integrate_f90_dev for method ICNTRL(3) = icntrl3 */
+size_t kpp_workspace_bytes_f90(const char* mech, int icntrl3, const kpp_launch* l, int ncells);
+/* the parameter checks of int/rosenbrock.f90:306-433 after INTEGRATE's merge */
+int kpp_check_params_f90(const int ICNTRL_U[20], const double RCNTRL_U[20], const double* atol,
+                         const double* rtol, int nvar);
+
+/* ---- the C SDIRK integrator (int/sdirk.c) on the same device functions ----
+ * kpp_sdirk_dev replaces
+ *   int SDIRK(int N, KPP_REAL Tinitial, KPP_REAL Tfinal, KPP_REAL Y[],
+ *             KPP_REAL RelTol[], KPP_REAL AbsTol[], KPP_REAL RCNTRL[],
+ *             int ICNTRL[], KPP_REAL RSTATUS[], int ISTATUS[])   int/sdirk.c:170
+ * (and its INTEGRATE, :91-166) for every cell of a device batch, one lane per
+ * cell, over [b->tstart, b->tend] (or tstart0/tend0):
+ *  - ICNTRL/RCNTRL as SDIRK takes them (0-based, no merge): ICNTRL[1] 0 =
+ *    vector tolerances, ICNTRL[2] method 0/1 Sdirk2a, 2 Sdirk2b, 3 Sdirk3a,
+ *    4 Sdirk4a, 5 Sdirk4b (any other: Sdirk2a), [3] max steps (0: 200000),
+ *    [4] max Newton iterations (0: 8), [5] 0 = extrapolated Newton start;
+ *    RCNTRL[0..10] Hmin, Hmax, Hstart, FacMin (0.2), FacMax (10), FacRej
+ *    (0.1), FacSafe (0.9), ThetaMin (1e-3), NewtonTol (3e-2), Qmin (1),
+ *    Qmax (1.2);
+ *  - b->sun_mode, b->hstart (overrides RCNTRL[2]) and b->rconst as for
+ *    kpp_rosenbrock_dev; b->istat [8][ld] = ISTATUS[1..8] (Nfun..Nsng);
+ *    b->rstat must hold [3][ld] = RSTATUS[1..3] (Texit, Hexit, Hnew);
+ *  - the reference's SDIRK_PrepareMatrix halves only its local step on a
+ *    singular matrix and never reports -8; so does this one;
+ *  - deviations: parameter errors -1..-5 are returned (the reference prints
+ *    them and integrates with the setting uninitialised), failures -6/-7 are
+ *    returned per cell (the reference returns 0), -9/-10 as for Rosenbrock.
+ * Not built for the largest mechanisms (LU_NONZERO > 4096): KPP_E_ARG.
+ * Asynchronous on `stream`; the workspace holds kpp_workspace_bytes_sdirk(). */
+size_t kpp_workspace_bytes_sdirk(const char* mech, int icntrl2, int ncells);
+int kpp_sdirk_dev(const char* mech, const kpp_batch* b, const int ICNTRL[20],
+                  const double RCNTRL[20], const double* atol, const double* rtol,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- the C fully implicit Runge-Kutta integrator (int/runge_kutta.c) ----
+ * kpp_rk_dev replaces
+ *   void RungeKutta(int N, KPP_REAL T, KPP_REAL Tend, KPP_REAL Y[],
+ *                   KPP_REAL RelTol[], KPP_REAL AbsTol[], KPP_REAL RCNTRL[],
+ *                   int ICNTRL[], KPP_REAL RSTATUS[], int ISTATUS[],
+ *                   int* IERR)                         int/runge_kutta.c:203
+ * (and its INTEGRATE, :114-200) for every cell of a device batch, one lane
+ * per cell, over [b->tstart, b->tend] (or tstart0/tend0), with the complex
+ * sparse LU of util/sutil.c:28-87 on the device:
+ *  - ICNTRL/RCNTRL as RungeKutta takes them (0-based, no merge): ICNTRL[1]
+ *    0 = vector tolerances, [2] method 0/1 Radau-2A, 2 Lobatto-3C, 3 Gauss,
+ *    4 Radau-1A (the reference's switch), [3] max steps (0: 200000), [4] max
+ *    Newton iterations (0: 8), [5] 0 = extrapolated Newton start, [9] 1 =
+ *    SDIRK error estimator (INTEGRATE's choice), 0 classical, [10] 0 =
+ *    Gustafsson's predictive controller; RCNTRL[0..10] Hmin, Hmax, Hstart,
+ *    FacMin (0.2), FacMax (8), FacRej (0.1), FacSafe (0.9), ThetaMin (1e-3),
+ *    NewtonTol (3e-2), Qmin (1), Qmax (1.2);
+ *  - b->sun_mode, b->hstart (overrides RCNTRL[2]) and b->rconst as for
+ *    kpp_rosenbrock_dev; b->istat [8][ld] = ISTATUS[0..7] (Nfun..Nsng, the
+ *    reference's counting: RK_PrepareRHS's evaluations and the solves of the
+ *    error estimators are not counted); b->rstat must hold [3][ld] =
+ *    RSTATUS[0..2] (Texit, Hacc, Hnew);
+ *  - every parameter check runs and the last failing one's code, -1..-8 or
+ *    -13, is returned (every cell's ierr set to it), as the reference's IERR;
+ *  - the -10 (step size) and -12 (five singular matrices in a row)
+ *    conditions are only reported by the reference, which goes on; so does
+ *    this one;
+ *  - deviations: -9 (more than ICNTRL[3] steps) ends the cell (the
+ *    reference prints it and goes on), a NaN error estimate or non-finite
+ *    step ends it with -14, more than 100000 passes in a row in which T does
+ *    not advance with -15; ICNTRL[2] = 5 (Lobatto-3A, "not yet implemented"
+ *    in the reference) is -13.
+ * Not built for the largest mechanisms (LU_NONZERO > 4096): KPP_E_ARG.
+ * Asynchronous on `stream`; the workspace holds kpp_workspace_bytes_rk(). */
+size_t kpp_workspace_bytes_rk(const char* mech, int ncells);
+int kpp_rk_dev(const char* mech, const kpp_batch* b, const int ICNTRL[20],
+               const double RCNTRL[20], const double* atol, const double* rtol, void* workspace,
+               size_t workspace_bytes, void* stream);
+
+/* ---- the C discrete adjoint Rosenbrock integrator (int/rosenbrock_adj.c) ----
+ * kpp_rosenbrock_adj_dev replaces
+ *   int RosenbrockADJ(KPP_REAL Y[], int NADJ, KPP_REAL Lambda[][NVAR],
+ *                     KPP_REAL Tstart, KPP_REAL Tend, KPP_REAL AbsTol[],
+ *                     KPP_REAL RelTol[], KPP_REAL AbsTol_adj[][NVAR],
+ *                     KPP_REAL RelTol_adj[][NVAR], KPP_REAL RCNTRL[],
+ *                     int ICNTRL[], KPP_REAL RSTATUS[], int ISTATUS[])
+ *                                                  int/rosenbrock_adj.c:191
+ * (and its INTEGRATE_ADJ, :116-188) for every cell of a device batch, one
+ * lane per cell, forward then backward, the adjoint's model functions
+ * (Hessian, HessTR_Vec, JacTR_SP_Vec, KppSolveTR) on the device:
+ *  - ICNTRL/RCNTRL as RosenbrockADJ takes them (0-based, no merge):
+ *    ICNTRL[0] autonomous, [1] 0 = vector tolerances, [2] method 0/4 Rodas3,
+ *    1 Ros2, 2 Ros3, 3 Ros4, 5 Rodas4, 7 Rodas3_1 (the reference's switch and
+ *    tables), [3] must be 0 (the reference's check of it always fails), [5]
+ *    CadjMethod (checked, unused), [6] AdjointType 0/2 discrete, 1 none, 3/4
+ *    continuous (not built: -13), [7] SaveLU; RCNTRL[0..6] Hmin, Hmax,
+ *    Hstart (0: max(Hmin, 1e-5)), FacMin (0.2), FacMax (6), FacRej (0.1),
+ *    FacSafe (0.9); AbsTol_adj/RelTol_adj only serve the continuous types;
+ *  - lambda [nadj*NVAR][b->ld] (row a*NVAR + i is Lambda[a][i]) holds
+ *    dg_a/dy(tend) on entry and dg_a/dy(tstart) on return; nadj may be 0;
+ *  - nchk checkpoint slots per cell (the reference's bufsize is 200000): a
+ *    cell with more accepted steps ends with -12 (the reference exits);
+ *  - first_step 0 keeps the reference's backward loop (while stack_ptr > 0),
+ *    which leaves out the first accepted step; 1 differentiates every step
+ *    (the F90 integrator's stack, int/rosenbrock_adj.f90);
+ *  - b->sun_mode and b->rconst as for kpp_rosenbrock_dev (b->hstart is not
+ *    read); b->istat [8][ld] = ISTATUS[1..8] (Nfun..Nsng, backward pass
+ *    included); b->rstat must hold [3][ld] = RSTATUS[1..3] (Texit, Hexit,
+ *    Hnew);
+ *  - parameter errors (-1..-5, -9, -13; the first failing check in the
+ *    reference's order) set every cell's ierr and are returned;
+ *  - deviations: a NaN error estimate or non-finite step ends the cell with
+ *    -10, more than 100000 rejections in a row with -11.
+ * Built for mechanisms compiled with the adjoint's functions (NVAR <= 300)
+ * whose LU_NONZERO <= 4096, else KPP_E_ARG.  Asynchronous on `stream`; the
+ * workspace holds kpp_workspace_bytes_adj() (0: not built or bad method). */
+size_t kpp_workspace_bytes_adj(const char* mech, int ncells, const int ICNTRL[20], int nadj,
+                               int nchk);
+int kpp_rosenbrock_adj_dev(const char* mech, const kpp_batch* b, const int ICNTRL[20],
+                           const double RCNTRL[20], const double* atol, const double* rtol,
+                           int nadj, double* lambda, int nchk, int first_step, void* workspace,
+                           size_t workspace_bytes, void* stream);
 
 /* ---- batched generated-model functions (device SoA arrays, ld >= n) ---- */
 /* Fun(V,F,RCT,Vdot)                 src/gen.c:642-760 */
@@ -226,6 +390,18 @@ int kpp_get_state(kpp_ctx* ctx, double* C, int* ierr, int* istat, double* rstat)
  * cell carried from the previous call's Hexit. */
 int kpp_ctx_set_tol(kpp_ctx* ctx, const double* atol, const double* rtol, int vector_tol);
 int INTEGRATE_BATCH(kpp_ctx* ctx, double TIN, double TOUT);
+/* The F90 INTEGRATE(TIN, TOUT, ICNTRL_U, RCNTRL_U, ISTATUS_U, RSTATUS_U,
+ * IERR_U) (int/rosenbrock.f90:43-125) for every cell, through
+ * kpp_integrate_f90_dev: the context's state, tolerances (vector tolerances
+ * are required when ICNTRL(2) = 0) and passive species; host outputs, each
+ * optional: ISTATUS_U [8][ncells], RSTATUS_U [3][ncells], IERR_U [ncells].
+ * Synchronous.  Not with KPP_CTX_MIXED; the learned scheduler is not used. */
+int INTEGRATE_F90_BATCH(kpp_ctx* ctx, double TIN, double TOUT, const int ICNTRL_U[20],
+                        const double RCNTRL_U[20], int* ISTATUS_U, double* RSTATUS_U,
+                        int* IERR_U);
+/* passive species of INTEGRATE_F90_BATCH: host [NVAR], nonzero = passive, or
+ * NULL for none (NonPassiveSpc_Indices of the F90 Initialize) */
+int kpp_ctx_set_passive(kpp_ctx* ctx, const int* passive);
 
 /* ---- autotuner (no counterpart in KPP) ----
  * Times candidate launch shapes of the persistent kernel (cells per

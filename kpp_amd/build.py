@@ -48,6 +48,12 @@ MECHANISMS = {
     # path, so not under KPP_HOME): six independent linear species whose
     # steps and singular iteration matrices have closed forms
     "linear6": (str(PKG / "mechs" / "linear6.kpp"), []),
+    # this project's #FAMILIES test mechanism (kpp_amd/mechs/strato_fam.kpp):
+    # small_strato with production/loss families of odd oxygen
+    "strato_fam": (str(PKG / "mechs" / "strato_fam.kpp"), []),
+    # KPP's GEOS-Chem carbon-gases example (models/carbon.*): F90 rate
+    # expressions (EXP, D exponents), fixed OH/Cl and diagnostic species
+    "carbon": ("models/carbon.def", ["--rates-f90"]),
     "mcm": ("examples/mcm/mcm.kpp",
             ["--no-pipelined", "--part-ops", "3000", "--rates-f90", "--f90-module", "examples/mcm/constants_mcm.f90",
              "--env", "M = CFACTOR*1.0D6; O2 = 0.21*M; N2 = 0.78*M; H2O = 1.0E-2*M; "
@@ -74,7 +80,8 @@ HIP_FLAGS = [
 # cells fill one wavefront per SIMD at most), 2.03 s at 2 and 1.88 s at 1
 # (VGPRs plus AGPRs for spills).  small_strato: 2, so that its register
 # kernel (the whole working set in VGPRs, 71-86 doubles per lane) fits.
-WAVES_PER_EU = {"small_strato": 2, "saprc99": 2, "saprcnov": 2, "linear6": 2, "mcm": 1}
+WAVES_PER_EU = {"small_strato": 2, "saprc99": 2, "saprcnov": 2, "linear6": 2, "strato_fam": 2,
+                "carbon": 2, "mcm": 1}
 
 
 def kpp_home() -> Path:
@@ -122,25 +129,54 @@ def generate(mechs=None) -> list[str]:
         (GEN / f"kpphip_{root}_comp.hip").write_text(
             f'#include "{root}_hip.h"\n#include "mech_launch.h"\n'
             f'KPPHIP_DEFINE_MECH_COMPONENTS({root})\n')
+        (GEN / f"kpphip_{root}_sdirk.hip").write_text(
+            f'#include "{root}_hip.h"\n#include "implicit_launch.h"\n'
+            f'KPPHIP_DEFINE_MECH_SDIRK({root})\n')
+        (GEN / f"kpphip_{root}_rk.hip").write_text(
+            f'#include "{root}_hip.h"\n#include "implicit_launch.h"\n'
+            f'KPPHIP_DEFINE_MECH_RK({root})\n')
+        (GEN / f"kpphip_{root}_adj.hip").write_text(
+            f'#include "{root}_hip.h"\n#include "adj_launch.h"\n'
+            f'KPPHIP_DEFINE_MECH_ADJ({root})\n')
         done.append(root)
     (GEN / "kpphip_registry.inc").write_text("".join(f"KPPHIP_MECH({r})\n" for r in done))
     return done
 
 
 def _mech_of(src: Path) -> str:
-    return src.stem.replace("kpphip_", "", 1).replace("_mixed", "").replace("_comp", "")
+    stem = src.stem.replace("kpphip_", "", 1)
+    for suffix in ("_mixed", "_comp", "_sdirk", "_rk", "_adj"):
+        if stem.endswith(suffix):
+            return stem[:-len(suffix)]
+    return stem
+
+
+def _includes(src: Path) -> list[Path]:
+    """The project headers a source includes, directly or not (quoted
+    #include lines resolved in its own folder, csrc, _gen and include)."""
+    import re
+    seen, todo = [], [src]
+    dirs = [CSRC, GEN, REPO / "include"]
+    while todo:
+        f = todo.pop()
+        for name in re.findall(r'^\s*#\s*include\s+"([^"]+)"', f.read_text(), re.M):
+            for d in (f.parent, *dirs):
+                p = d / name
+                if p.exists():
+                    if p not in seen:
+                        seen.append(p)
+                        todo.append(p)
+                    break
+    return sorted(seen)
 
 
 def _stamp(src: Path, cmd: list) -> str:
-    """Digest of a translation unit's inputs: the command, the source, the
-    library's headers and the mechanism's generated header."""
+    """Digest of a translation unit's inputs: the command, the source and
+    every project header it includes."""
     import hashlib
     h = hashlib.sha256(" ".join(cmd).encode())
-    deps = [src, *sorted(CSRC.glob("*.h")), *sorted((REPO / "include").glob("*.h"))]
-    mh = GEN / (_mech_of(src) + "_hip.h")
-    if mh.exists():
-        deps.append(mh)
-    for d in deps:
+    for d in [src, *_includes(src)]:
+        h.update(str(d).encode())
         h.update(d.read_bytes())
     return h.hexdigest()
 
@@ -167,6 +203,9 @@ def build_lib(mechs: list[str], jobs: int = 8) -> Path:
     srcs = ([GEN / f"kpphip_{m}.hip" for m in mechs] +
             [GEN / f"kpphip_{m}_mixed.hip" for m in mechs] +
             [GEN / f"kpphip_{m}_comp.hip" for m in mechs] +
+            [GEN / f"kpphip_{m}_sdirk.hip" for m in mechs] +
+            [GEN / f"kpphip_{m}_rk.hip" for m in mechs] +
+            [GEN / f"kpphip_{m}_adj.hip" for m in mechs] +
             [CSRC / "kpphip_api.cpp", CSRC / "mlp.hip", CSRC / "sched.cpp"])
     # largest mechanism first, so the slowest translation units start at once
     def weight(src):

@@ -216,6 +216,7 @@ struct Mech {
   // stoichiometry [species appearance index][equation], float like KPP
   std::vector<std::vector<float>> sl, sr, st;
   std::vector<Rate> rates;
+  std::vector<std::string> families;  // #FAMILIES names, in declaration order
 
   SpeciesDef& spc(int c) {
     if (c == CODE_HV) return hv;
@@ -243,6 +244,15 @@ std::vector<std::string> split_semis(const std::string& body) {
     }
   }
   if (!trim(cur).empty()) out.push_back(cur);
+  return out;
+}
+
+std::vector<std::string> split_plus(const std::string& body) {
+  std::vector<std::string> out(1);
+  for (char c : body) {
+    if (c == '+') out.emplace_back();
+    else out.back() += c;
+  }
   return out;
 }
 
@@ -404,6 +414,83 @@ void parse_equation(Mech& m, const std::string& text) {
     }
   }
   m.rates.push_back(r);
+}
+
+// #FAMILIES: production/loss families (src/scan.y:244-268, src/scanner.c:
+// 921-1097).  "Pname : m1 + 2 m2;" or "Lname : ...": for every equation
+// parsed so far, the family's loss L = sum coeff*Stoich_Left and production
+// P = sum coeff*Stoich_Right over its members (floats, as the reference's
+// Loss_Coeff/Prod_Coeff); where P - L (production family) or L - P (loss
+// family) is positive, a variable species named after the family (declared
+// on first use) is made a product of the equation with that coefficient,
+// printed with "%f" and read back as the reference does.  A member's
+// coefficient is truncated to an integer (the reference stores
+// (unsigned char)atof(nr)).  Families declared before the equations see no
+// equations, as in the reference.
+void declare_families(Mech& m, const std::string& body) {
+  for (auto& item : split_semis(body)) {
+    std::string s = trim(item);
+    if (s.empty()) continue;
+    auto colon = s.find(':');
+    if (colon == std::string::npos) die("bad #FAMILIES entry: " + s);
+    std::string fam = trim(s.substr(0, colon));
+    for (auto& f : m.families)
+      if (eq_nocase(f, fam)) die("multiple declaration for family " + fam);
+    int type;
+    if (!fam.empty() && fam[0] == 'P') type = 1;       // PROD_FAM
+    else if (!fam.empty() && fam[0] == 'L') type = 2;  // LOSS_FAM
+    else die(fam + " must be declared as a P/L term with appropriate prefix (P,L)");
+    const int neq = (int)m.rates.size();
+    std::vector<float> loss(neq, 0.f), prod(neq, 0.f);
+    std::string rest = s.substr(colon + 1);
+    for (auto& term : split_plus(rest)) {
+      std::string t = trim(term), nr = "1", name = t;
+      size_t p = 0;
+      while (p < t.size() && (isdigit((unsigned char)t[p]) || t[p] == '.')) p++;
+      if (p > 0) {
+        nr = t.substr(0, p);
+        name = trim(t.substr(p));
+      }
+      const int c = m.find_species(name);
+      if (c < 0) die("undefined member " + name + " of family " + fam);
+      const float coeff = (float)(unsigned char)atof(nr.c_str());
+      auto it = m.reverse.find(c);
+      if (it == m.reverse.end()) continue;  // in no equation
+      const int k = it->second;
+      for (int i = 0; i < neq; i++) {
+        if (m.sl[k][i] > 0) loss[i] += m.sl[k][i] * coeff;
+        if (m.sr[k][i] > 0) prod[i] += m.sr[k][i] * coeff;
+      }
+    }
+    m.families.push_back(fam);
+    for (int i = 0; i < neq; i++) {
+      const float net = (type == 1) ? prod[i] - loss[i] : loss[i] - prod[i];
+      if (!(net > 0.f)) continue;
+      int c = m.find_species(fam);
+      if (c < 0) {
+        m.table.push_back({fam, VAR_SPC, ""});
+        c = (int)m.table.size() - 1;
+      }
+      int k;
+      auto it = m.reverse.find(c);
+      if (it == m.reverse.end()) {
+        k = (int)m.code.size();
+        m.code.push_back(c);
+        m.reactive.push_back(0);
+        m.reverse[c] = k;
+        m.sl.emplace_back(neq, 0.f);
+        m.sr.emplace_back(neq, 0.f);
+        m.st.emplace_back(neq, 0.f);
+      } else {
+        k = it->second;
+      }
+      char buf[64];
+      snprintf(buf, sizeof buf, "+%f", (double)net);
+      const double val = strtod(buf, nullptr);
+      m.sr[k][i] = (float)val;
+      m.st[k][i] = (float)val;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1175,7 +1262,18 @@ struct Model {
   std::vector<std::string> b_expr;
   std::vector<std::string> jvs_expr;  // per LU entry
   std::vector<char> jvs_zero;
-  Ops fun_ops, jac_ops, lu_ops, sol_ops;
+  // Fun_SPLIT (#FUNCTION SPLIT): A[j] of the reactions producing a variable,
+  // production P[i] and destruction frequency D[i]
+  bool fun_split = false;
+  // the adjoint's model functions (src/gen.c:1327-1600, 1770-1870, 2035-2106)
+  bool adjoint = false;
+  std::vector<std::string> d2a_expr, hess_expr, htu_expr, juv_expr, jtuv_expr;
+  std::vector<int> ih_i, ih_j, ih_k;
+  Ops hess_ops;
+  void build_adjoint();
+  std::vector<int> as_used;
+  std::vector<std::string> as_expr, p_expr, d_expr;
+  Ops fun_ops, jac_ops, lu_ops, sol_ops, split_ops;
   std::vector<std::pair<int, std::string>> rconst;  // (reaction, C expression)
   // rate classes: 0 = fixed per cell, 1 = SUN/TIME, 2 = state (concentrations)
   std::map<int, int> rate_class;
@@ -1229,6 +1327,51 @@ struct Model {
         sum = Add(sum, Mul(Const(m->st[i][j]), Ref("A[" + std::to_string(j) + "]")));
       vdot_expr.push_back(pf.expr(sum));
     }
+    if (fun_split) {
+      // Fun_SPLIT (src/gen.c:713-812, z_useAggregate = 0): A[j] for the
+      // reactions with a variable product; P = sum Stoich_Right*A; D = sum
+      // over the reactions consuming i of RCT*Stoich_Left times the other
+      // reactants (the loss frequency); Vdot = P - D*V
+      Printer ps{&split_ops};
+      for (int j = 0; j < ne; j++) {
+        bool used = false;
+        for (int i = 0; i < nv; i++)
+          if (m->sr[i][j] != 0) { used = true; break; }
+        if (!used) continue;
+        Node* p = rconst_node(j);
+        int i = 0;
+        for (; i < nv; i++)
+          for (int k = 1; k <= (int)m->sl[i][j]; k++) p = Mul(p, Ref("V[" + std::to_string(i) + "]"));
+        for (; i < ns; i++)
+          for (int k = 1; k <= (int)m->sl[i][j]; k++)
+            p = Mul(p, Ref("F[" + std::to_string(i - nv) + "]"));
+        as_used.push_back(j);
+        as_expr.push_back(ps.expr(p));
+      }
+      for (int i = 0; i < nv; i++) {
+        Node* sum = Const(0);
+        for (int j = 0; j < ne; j++)
+          sum = Add(sum, Mul(Const(m->sr[i][j]), Ref("A[" + std::to_string(j) + "]")));
+        p_expr.push_back(ps.expr(sum));
+      }
+      for (int i = 0; i < nv; i++) {
+        Node* sum = Const(0);
+        for (int j = 0; j < ne; j++) {
+          if (m->sl[i][j] == 0) continue;
+          Node* p = Mul(rconst_node(j), Const(m->sl[i][j]));
+          int l = 0;
+          for (; l < nv; l++) {
+            int mm = (int)m->sl[l][j] - (l == i);
+            for (int k = 1; k <= mm; k++) p = Mul(p, Ref("V[" + std::to_string(l) + "]"));
+          }
+          for (; l < ns; l++)
+            for (int k = 1; k <= (int)m->sl[l][j]; k++)
+              p = Mul(p, Ref("F[" + std::to_string(l - nv) + "]"));
+          sum = Add(sum, p);
+        }
+        d_expr.push_back(ps.expr(sum));
+      }
+    }
     // Jac_SP (src/gen.c:1239-1284)
     Printer pj{&jac_ops};
     for (int i = 0; i < ne; i++)
@@ -1278,6 +1421,150 @@ struct Model {
       if (m->rates[j].type != NUMBER) rconst.push_back({j, m->rates[j].expr});
   }
 };
+
+// The model functions of the discrete adjoint (int/rosenbrock_adj.c).
+//  Hessian (src/gen.c:1327-1498): D2A[k] = d2 A_j/dV_i1 dV_i2 (i1 <= i2) for
+//   every reaction with the reactants, then HESS[m] = d2 f_i/dV_j dV_k =
+//   sum_j Stoich[i][j] D2A, entries (i, j <= k) in that loop order;
+//  HessTR_Vec (:1527-1549): HTU_i = sum over entries with j == i of
+//   HESS*(U1[i_]*U2[k]), and with k == i of HESS*(U1[i_]*U2[j]);
+//  Jac_SP_Vec / JacTR_SP_Vec (src/gen.c:1770-1870): J*U and J^T*U over the
+//   Jacobian's structural entries (structJ) stored in the LU pattern.
+void Model::build_adjoint() {
+  const int nv = S.nvar, ns = S.nspc, ne = S.nreact;
+  Printer ph{&hess_ops};
+  std::map<std::tuple<int, int, int>, int> d2a_of;  // (j, i1, i2) -> D2A index
+  auto V = [](int i) { return Ref("V[" + std::to_string(i) + "]"); };
+  auto Fx = [&](int i) { return Ref("F[" + std::to_string(i - nv) + "]"); };
+  for (int j = 0; j < ne; j++)
+    for (int i1 = 0; i1 < nv; i1++)
+      for (int i2 = i1; i2 < nv; i2++) {
+        Node* p = nullptr;
+        if (i1 == i2) {
+          if (m->sl[i1][j] < 2) continue;
+          p = rconst_node(j);
+          for (int i = 0; i < i1; i++)
+            for (int k = 1; k <= (int)m->sl[i][j]; k++) p = Mul(p, V(i));
+          p = Mul(p, Const(m->sl[i1][j]));
+          p = Mul(p, Const(m->sl[i1][j] - 1));
+          for (int k = 1; k <= (int)m->sl[i1][j] - 2; k++) p = Mul(p, V(i1));
+          int i = i1 + 1;
+          for (; i < nv; i++)
+            for (int k = 1; k <= (int)m->sl[i][j]; k++) p = Mul(p, V(i));
+          for (; i < ns; i++)
+            for (int k = 1; k <= (int)m->sl[i][j]; k++) p = Mul(p, Fx(i));
+        } else {
+          if (!(m->sl[i1][j] >= 1 && m->sl[i2][j] >= 1)) continue;
+          p = rconst_node(j);
+          for (int i = 0; i < i1; i++)
+            for (int k = 1; k <= (int)m->sl[i][j]; k++) p = Mul(p, V(i));
+          p = Mul(p, Const(m->sl[i1][j]));
+          for (int k = 1; k <= (int)m->sl[i1][j] - 1; k++) p = Mul(p, V(i1));
+          for (int i = i1 + 1; i < i2; i++)
+            for (int k = 1; k <= (int)m->sl[i][j]; k++) p = Mul(p, V(i));
+          p = Mul(p, Const(m->sl[i2][j]));
+          for (int k = 1; k <= (int)m->sl[i2][j] - 1; k++) p = Mul(p, V(i2));
+          int i = i2 + 1;
+          for (; i < nv; i++)
+            for (int k = 1; k <= (int)m->sl[i][j]; k++) p = Mul(p, V(i));
+          for (; i < ns; i++)
+            for (int k = 1; k <= (int)m->sl[i][j]; k++) p = Mul(p, Fx(i));
+        }
+        d2a_of[{j, i1, i2}] = (int)d2a_expr.size();
+        d2a_expr.push_back(ph.expr(p));
+      }
+  // reactions per (i1, i2) pair, in j order
+  std::map<std::pair<int, int>, std::vector<std::pair<int, int>>> by_pair;
+  for (auto& kv : d2a_of)
+    by_pair[{std::get<1>(kv.first), std::get<2>(kv.first)}].push_back(
+        {std::get<0>(kv.first), kv.second});
+  for (auto& kv : by_pair) std::sort(kv.second.begin(), kv.second.end());
+  for (int i = 0; i < nv; i++)
+    for (auto& kv : by_pair) {  // (i1, i2) in lexicographic order, as the loops
+      Node* sum = Const(0);
+      bool any = false;
+      for (auto& jk : kv.second)
+        if (m->st[i][jk.first] != 0) {
+          sum = Add(sum, Mul(Const(m->st[i][jk.first]), Ref("D2A[" + std::to_string(jk.second) + "]")));
+          any = true;
+        }
+      if (!any) continue;
+      ih_i.push_back(i);
+      ih_j.push_back(kv.first.first);
+      ih_k.push_back(kv.first.second);
+      hess_expr.push_back(ph.expr(sum));
+    }
+  const int nh = (int)hess_expr.size();
+  for (int i = 0; i < nv; i++) {
+    Node* sum = Const(0);
+    for (int k = 0; k < nh; k++) {
+      const std::string h = "HESS[" + std::to_string(k) + "]";
+      if (ih_j[k] == i)
+        sum = Add(sum, Mul(Ref(h), Mul(Ref("U1[" + std::to_string(ih_i[k]) + "]"),
+                                       Ref("U2[" + std::to_string(ih_k[k]) + "]"))));
+      else if (ih_k[k] == i)
+        sum = Add(sum, Mul(Ref(h), Mul(Ref("U1[" + std::to_string(ih_i[k]) + "]"),
+                                       Ref("U2[" + std::to_string(ih_j[k]) + "]"))));
+    }
+    htu_expr.push_back(ph.expr(sum));
+  }
+  std::map<std::pair<int, int>, int> lu_at;
+  for (int r = 0; r < nv; r++)
+    for (int q = S.lu_crow[r]; q < S.lu_crow[r + 1]; q++) lu_at[{r, S.lu_icol[q]}] = q;
+  for (int i = 0; i < nv; i++) {
+    Node* sum = Const(0);
+    for (int j = 0; j < nv; j++)
+      if (S.LU[i][j] && S.J[i][j] != 0)
+        sum = Add(sum, Mul(Ref("JVS[" + std::to_string(lu_at[{i, j}]) + "]"),
+                           Ref("UV[" + std::to_string(j) + "]")));
+    juv_expr.push_back(ph.expr(sum));
+  }
+  for (int i = 0; i < nv; i++) {
+    Node* sum = Const(0);
+    for (int j = 0; j < nv; j++)
+      if (S.J[j][i] != 0)
+        sum = Add(sum, Mul(Ref("JVS[" + std::to_string(lu_at[{j, i}]) + "]"),
+                           Ref("UV[" + std::to_string(j) + "]")));
+    jtuv_expr.push_back(ph.expr(sum));
+  }
+}
+
+// KppSolveTR (src/gen.c:2035-2106): (LU)^T x = b, U^T forward then L^T
+// backward; X may alias XX.  Statements per row.
+std::vector<std::string> solve_tr_stmts(const Structure& S) {
+  const int nv = S.nvar;
+  std::map<std::pair<int, int>, int> at;  // (row, col) -> LU index
+  for (int r = 0; r < nv; r++)
+    for (int q = S.lu_crow[r]; q < S.lu_crow[r + 1]; q++) at[{r, S.lu_icol[q]}] = q;
+  std::vector<std::string> st;
+  Ops dummy;
+  Printer pr{&dummy};
+  for (int i = 0; i < nv; i++) {  // XX_i = (X_i - sum_{j<i} U_ji XX_j) / U_ii
+    Node* sum = Ref("X[" + std::to_string(i) + "]");
+    for (int j = 0; j < i; j++) {
+      auto it = at.find({j, i});
+      if (it != at.end())
+        sum = Sub(sum, Mul(Ref("JVS[" + std::to_string(it->second) + "]"),
+                           Ref("XX[" + std::to_string(j) + "]")));
+    }
+    sum = Div(sum, Ref("JVS[" + std::to_string(S.lu_diag[i]) + "]"));
+    st.push_back("  XX[" + std::to_string(i) + "] = " + pr.expr(sum) + ";\n");
+  }
+  for (int i = nv - 1; i >= 0; i--) {  // XX_i -= sum_{j>i} L_ji XX_j
+    Node* sum = Ref("XX[" + std::to_string(i) + "]");
+    bool any = false;
+    for (int j = i + 1; j < nv; j++) {
+      auto it = at.find({j, i});
+      if (it != at.end()) {
+        sum = Sub(sum, Mul(Ref("JVS[" + std::to_string(it->second) + "]"),
+                           Ref("XX[" + std::to_string(j) + "]")));
+        any = true;
+      }
+    }
+    if (any) st.push_back("  XX[" + std::to_string(i) + "] = " + pr.expr(sum) + ";\n");
+  }
+  return st;
+}
 
 bool mentions(const std::string& expr, const std::string& ident) {
   size_t p = 0;
@@ -1435,6 +1722,18 @@ void emit_c_model(Model& md, const std::string& path) {
     o << "  A[" << md.a_used[q] << "] = " << md.a_expr[q] << ";\n";
   for (int i = 0; i < S.nvar; i++) o << "  Vdot[" << i << "] = " << md.vdot_expr[i] << ";\n";
   o << "}\n";
+  o << "#define KPP_FUN_SPLIT " << (md.fun_split ? 1 : 0) << "\n";
+  if (md.fun_split) {
+    o << "/* Fun_SPLIT (src/gen.c:713-812): Vdot = P - D*V */\n";
+    o << "static void kpp_Fun_SPLIT(const double* V, const double* F, const double* RCT, "
+         "double* Vdot) {\n  double A[" << std::max(S.nreact, 1) << "], P[" << S.nvar << "], D["
+      << S.nvar << "];\n  (void)F; (void)RCT;\n";
+    for (size_t q = 0; q < md.as_used.size(); q++)
+      o << "  A[" << md.as_used[q] << "] = " << md.as_expr[q] << ";\n";
+    for (int i = 0; i < S.nvar; i++) o << "  P[" << i << "] = " << md.p_expr[i] << ";\n";
+    for (int i = 0; i < S.nvar; i++) o << "  D[" << i << "] = " << md.d_expr[i] << ";\n";
+    o << "  for (int n = 0; n < " << S.nvar << "; n++) Vdot[n] = P[n] - D[n]*V[n];\n}\n";
+  }
   o << "static void kpp_Jac_SP(const double* V, const double* F, const double* RCT, double* "
        "JVS) {\n  double B[" << std::max<size_t>(md.b_expr.size(), 1)
     << "];\n  (void)V; (void)F; (void)RCT;\n";
@@ -1462,6 +1761,43 @@ void emit_c_model(Model& md, const std::string& path) {
     }
   }
   o << "}\n";
+  o << "#define KPP_ADJOINT " << (md.adjoint ? 1 : 0) << "\n";
+  if (md.adjoint) {
+    const size_t nh = md.hess_expr.size();
+    o << "/* the discrete adjoint's model functions (src/gen.c:1327-1600, 1770-1870,\n"
+         "   2035-2106): HESS(m) = d2 f_i/dV_j dV_k, i = IHESS_I(m), j <= k */\n";
+    o << "#define KPP_NHESS " << std::max<size_t>(nh, 1) << "\n";
+    auto arr = [&](const char* name, const std::vector<int>& v) {
+      o << "static const int " << name << "[" << std::max<size_t>(v.size(), 1) << "] = {";
+      for (size_t q = 0; q < v.size(); q++) o << (q ? "," : "") << v[q];
+      if (v.empty()) o << "0";
+      o << "};\n";
+    };
+    arr("kpp_IHESS_I", md.ih_i);
+    arr("kpp_IHESS_J", md.ih_j);
+    arr("kpp_IHESS_K", md.ih_k);
+    o << "static void kpp_Hessian(const double* V, const double* F, const double* RCT, double* "
+         "HESS) {\n  double D2A[" << std::max<size_t>(md.d2a_expr.size(), 1)
+      << "];\n  (void)V; (void)F; (void)RCT; (void)D2A;\n";
+    for (size_t q = 0; q < md.d2a_expr.size(); q++)
+      o << "  D2A[" << q << "] = " << md.d2a_expr[q] << ";\n";
+    for (size_t q = 0; q < nh; q++) o << "  HESS[" << q << "] = " << md.hess_expr[q] << ";\n";
+    if (nh == 0) o << "  HESS[0] = 0.0;\n";
+    o << "}\n";
+    o << "static void kpp_HessTR_Vec(const double* HESS, const double* U1, const double* U2, "
+         "double* HTU) {\n  (void)HESS; (void)U1; (void)U2;\n";
+    for (int i = 0; i < S.nvar; i++) o << "  HTU[" << i << "] = " << md.htu_expr[i] << ";\n";
+    o << "}\n";
+    o << "static void kpp_Jac_SP_Vec(const double* JVS, const double* UV, double* JUV) {\n";
+    for (int i = 0; i < S.nvar; i++) o << "  JUV[" << i << "] = " << md.juv_expr[i] << ";\n";
+    o << "}\n";
+    o << "static void kpp_JacTR_SP_Vec(const double* JVS, const double* UV, double* JTUV) {\n";
+    for (int i = 0; i < S.nvar; i++) o << "  JTUV[" << i << "] = " << md.jtuv_expr[i] << ";\n";
+    o << "}\n";
+    o << "static void kpp_KppSolveTR(const double* JVS, const double* X, double* XX) {\n";
+    for (auto& st : solve_tr_stmts(S)) o << st;
+    o << "}\n";
+  }
   o << "#undef ARR_abc\n#undef ARR_ab\n#undef ARR_ac\n#undef EP2\n#undef EP3\n#undef FALL\n";
   o << "#endif\n";
 }
@@ -1665,6 +2001,34 @@ void emit_hip(Model& md, const std::string& path) {
     << ";\n";
   o << "static constexpr int N_SUN_RATES = " << rc_sun.size() << ";\n";
   o << "static constexpr int N_STATE_RATES = " << rc_state.size() << ";\n";
+  // #FUNCTION SPLIT: the F90 integrators call Fun_SPLIT (src/code.c:279-280,
+  // 361-362: KPP_FUN_OR_FUN_SPLIT); the C integrators always call Fun
+  o << "static constexpr bool FUN_SPLIT = " << (md.fun_split ? "true" : "false") << ";\n";
+  if (md.fun_split) {
+    o << "// Fun_SPLIT (src/gen.c:713-812): Vdot = P - D*V\n";
+    o << "template <class TV, class TF, class TR, class TO>\nstatic __device__ __forceinline__ "
+         "void FunSplit(TV V, TF F, TR RCT, TO Vdot) {\n";
+    for (size_t q = 0; q < md.as_used.size(); q++)
+      o << "  const double A" << md.as_used[q] << " = " << md.as_expr[q] << ";\n";
+    auto locals = [](const std::string& e) {  // A[j] -> Aj
+      std::string r;
+      for (size_t p = 0; p < e.size(); p++) {
+        if (e[p] == 'A' && p + 1 < e.size() && e[p + 1] == '[' &&
+            (p == 0 || !(isalnum((unsigned char)e[p - 1]) || e[p - 1] == '_'))) {
+          size_t q = e.find(']', p);
+          r += "A" + e.substr(p + 2, q - p - 2);
+          p = q;
+        } else {
+          r += e[p];
+        }
+      }
+      return r;
+    };
+    for (int i = 0; i < S.nvar; i++)
+      o << "  { const double P = " << locals(md.p_expr[i]) << "; const double D = "
+        << md.d_expr[i] << "; Vdot[" << i << "] = P - D*V[" << i << "]; }\n";
+    o << "}\n";
+  }
   o << "#define KPPQ static __device__ __forceinline__\n" << kRateLawsBody << f90::kPowiHelpers
     << "#undef KPPQ\n";
   o << "// rate constants that depend on neither SUN, TIME nor the state (once per cell)\n";
@@ -2052,6 +2416,118 @@ void emit_hip(Model& md, const std::string& path) {
     routine("template <class TG, class TX, class TB, class TR>", "Residual",
             "TG G, TX X, TB B, TR R", "G, X, B, R", 0, st);
   }
+  // Iteration matrices from a stored Jacobian, for the integrators that keep
+  // it between factorisations (SDIRK_PrepareMatrix, int/sdirk.c:905-960;
+  // RK_Decomp, int/runge_kutta.c:1190-1246): G = -J with ghinv added to the
+  // diagonal (the reference's two loops, entry by entry the same values),
+  // and DiagFill for the imaginary part of RK_Decomp's complex matrix (v on
+  // the diagonal, zero elsewhere).
+  {
+    std::vector<std::pair<std::string, long>> st, sd;
+    for (int i = 0; i < S.nvar; i++) {
+      std::string t, d;
+      for (int k = S.lu_crow[i]; k < S.lu_crow[i + 1]; k++) {
+        const std::string n = std::to_string(k);
+        const bool diag = k == S.lu_diag[i];
+        t += "  G[" + n + "] = -J[" + n + "]" + (diag ? "+ghinv" : "") + ";\n";
+        d += "  G[" + n + "] = " + (diag ? "v" : "0.0") + ";\n";
+      }
+      st.push_back({t, (long)(S.lu_crow[i + 1] - S.lu_crow[i]) + 1});
+      sd.push_back({d, 1});
+    }
+    routine("template <class TJ, class TG>", "JacToGhimj", "TJ J, double ghinv, TG G",
+            "J, ghinv, G", 0, st);
+    routine("template <class TG>", "DiagFill", "TG G, double v", "G, v", 0, sd);
+  }
+  // KppDecompCmplxR (util/sutil.c:28-58), the complex elimination of
+  // RK_Decomp, unrolled like KppDecomp: row k's real and imaginary parts
+  // held in locals (wr<x>, wi<x>) and stored once at the end.  Per entry the
+  // reference's operations in its order: den = dR*dR + dI*dI,
+  // ar = -(wr_j*dR + wi_j*dI)/den, ai = -(wi_j*dR - wr_j*dI)/den,
+  // w_j = -a, then w_c = w_c + ar*U_c - ai*V_c (real) and
+  // w_c + ar*V_c + ai*U_c (imaginary).  Returns k+1 for the first row whose
+  // real or imaginary diagonal is exactly zero when its elimination starts.
+  {
+    std::vector<std::pair<std::string, long>> st;
+    for (int k = 0; k < S.nvar; k++) {
+      const std::string dk = std::to_string(S.lu_diag[k]), k1 = std::to_string(k + 1);
+      std::string t = "  if (GR[" + dk + "] == 0.0) return " + k1 + ";\n  if (GI[" + dk +
+                      "] == 0.0) return " + k1 + ";\n";
+      long n_ops = 0;
+      std::map<int, int> pos;
+      for (int kk = S.lu_crow[k]; kk < S.lu_crow[k + 1]; kk++) pos[S.lu_icol[kk]] = kk;
+      std::vector<int> touched;
+      std::string ops;
+      for (int kk = S.lu_crow[k]; kk < S.lu_diag[k]; kk++) {
+        const int j = S.lu_icol[kk];
+        const std::string d = std::to_string(S.lu_diag[j]), x = std::to_string(kk);
+        touched.push_back(kk);
+        ops += "    { const double den = GR[" + d + "]*GR[" + d + "] + GI[" + d + "]*GI[" + d +
+               "]; const double ar = -(wr" + x + "*GR[" + d + "] + wi" + x + "*GI[" + d +
+               "])/den; const double ai = -(wi" + x + "*GR[" + d + "] - wr" + x + "*GI[" + d +
+               "])/den; wr" + x + " = -ar; wi" + x + " = -ai;";
+        n_ops += 11;
+        for (int jj = S.lu_diag[j] + 1; jj < S.lu_crow[j + 1]; jj++) {
+          auto it = pos.find(S.lu_icol[jj]);
+          if (it == pos.end()) die("LU pattern not closed under elimination");
+          const std::string c = std::to_string(it->second), u = std::to_string(jj);
+          touched.push_back(it->second);
+          ops += " wr" + c + " = wr" + c + " + ar*GR[" + u + "] - ai*GI[" + u + "]; wi" + c +
+                 " = wi" + c + " + ar*GI[" + u + "] + ai*GR[" + u + "];";
+          n_ops += 8;
+        }
+        ops += " }\n";
+      }
+      if (!touched.empty()) {
+        std::sort(touched.begin(), touched.end());
+        touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+        t += "  {\n   ";
+        for (int x : touched)
+          t += " double wr" + std::to_string(x) + " = GR[" + std::to_string(x) + "], wi" +
+               std::to_string(x) + " = GI[" + std::to_string(x) + "];";
+        t += "\n" + ops + "   ";
+        for (int x : touched)
+          t += " GR[" + std::to_string(x) + "] = wr" + std::to_string(x) + "; GI[" +
+               std::to_string(x) + "] = wi" + std::to_string(x) + ";";
+        t += "\n  }\n";
+      }
+      st.push_back({t, n_ops + 2});
+    }
+    routine("template <class TG>", "KppDecompCmplxR", "TG GR, TG GI", "GR, GI", 1, st);
+  }
+  // KppSolveCmplxR (util/sutil.c:63-87): forward substitution with the unit
+  // lower factor, XR[i] -= (U*XR[c] - V*XI[c]) then XI[i] -= (U*XI[c] +
+  // V*XR[c]) per entry; backward with the running sums and the division by
+  // the complex diagonal written out as the reference does.
+  {
+    std::vector<std::pair<std::string, long>> st;
+    for (int i = 0; i < S.nvar; i++) {
+      const std::string is = std::to_string(i);
+      std::string t;
+      for (int j = S.lu_crow[i]; j < S.lu_diag[i]; j++) {
+        const std::string u = std::to_string(j), c = std::to_string(S.lu_icol[j]);
+        t += "  XR[" + is + "] = XR[" + is + "] - (GR[" + u + "]*XR[" + c + "] - GI[" + u +
+             "]*XI[" + c + "]);\n  XI[" + is + "] = XI[" + is + "] - (GR[" + u + "]*XI[" + c +
+             "] + GI[" + u + "]*XR[" + c + "]);\n";
+      }
+      if (!t.empty()) st.push_back({t, 8L * (S.lu_diag[i] - S.lu_crow[i])});
+    }
+    for (int i = S.nvar - 1; i >= 0; i--) {
+      const std::string is = std::to_string(i), d = std::to_string(S.lu_diag[i]);
+      std::string t = "  { double sr = XR[" + is + "], si = XI[" + is + "];";
+      for (int j = S.lu_diag[i] + 1; j < S.lu_crow[i + 1]; j++) {
+        const std::string u = std::to_string(j), c = std::to_string(S.lu_icol[j]);
+        t += " sr = sr - (GR[" + u + "]*XR[" + c + "] - GI[" + u + "]*XI[" + c + "]); si = si - (GR[" +
+             u + "]*XI[" + c + "] + GI[" + u + "]*XR[" + c + "]);";
+      }
+      t += " const double den = GR[" + d + "]*GR[" + d + "] + GI[" + d + "]*GI[" + d +
+           "]; XR[" + is + "] = (sr*GR[" + d + "] + si*GI[" + d + "])/den; XI[" + is +
+           "] = (si*GR[" + d + "] - sr*GI[" + d + "])/den; }\n";
+      st.push_back({t, 8L * (S.lu_crow[i + 1] - S.lu_diag[i] - 1) + 11});
+    }
+    routine("template <class TG, class TX>", "KppSolveCmplxR", "TG GR, TG GI, TX XR, TX XI",
+            "GR, GI, XR, XI", 0, st);
+  }
   for (int split = 0; split < (g_lds_split > 0 ? 2 : 1); split++) {
     auto jvs = [&](int x) {
       return split ? lget("JVS", x) : "JVS[" + std::to_string(x) + "]";
@@ -2085,6 +2561,37 @@ void emit_hip(Model& md, const std::string& path) {
               "TG JVS, TX X, TL Gs", "JVS, X, Gs", 0, st, "<NL>");
     else
       routine("template <class TG, class TX>", "KppSolve", "TG JVS, TX X", "JVS, X", 0, st);
+  }
+  // the discrete adjoint's model functions (Hessian, HessTR_Vec, Jac_SP_Vec,
+  // JacTR_SP_Vec, KppSolveTR), the oracle's expressions
+  o << "static constexpr bool ADJOINT = " << (md.adjoint ? "true" : "false") << ";\n";
+  if (md.adjoint) {
+    const size_t nh = md.hess_expr.size();
+    o << "static constexpr int NHESS = " << std::max<size_t>(nh, 1) << ";\n";
+    o << "template <class TV, class TF, class TR, class TH>\nstatic __device__ __forceinline__ "
+         "void Hessian(TV V, TF F, TR RCT, TH HESS) {\n  double D2A["
+      << std::max<size_t>(md.d2a_expr.size(), 1) << "];\n";
+    for (size_t q = 0; q < md.d2a_expr.size(); q++)
+      o << "  D2A[" << q << "] = " << md.d2a_expr[q] << ";\n";
+    for (size_t q = 0; q < nh; q++) o << "  HESS[" << q << "] = " << md.hess_expr[q] << ";\n";
+    if (nh == 0) o << "  HESS[0] = 0.0;\n";
+    o << "}\n";
+    o << "template <class TH, class T1, class T2, class TO>\nstatic __device__ __forceinline__ "
+         "void HessTR_Vec(TH HESS, T1 U1, T2 U2, TO HTU) {\n";
+    for (int i = 0; i < S.nvar; i++) o << "  HTU[" << i << "] = " << md.htu_expr[i] << ";\n";
+    o << "}\n";
+    o << "template <class TJ, class TU, class TO>\nstatic __device__ __forceinline__ "
+         "void Jac_SP_Vec(TJ JVS, TU UV, TO JUV) {\n";
+    for (int i = 0; i < S.nvar; i++) o << "  JUV[" << i << "] = " << md.juv_expr[i] << ";\n";
+    o << "}\n";
+    o << "template <class TJ, class TU, class TO>\nstatic __device__ __forceinline__ "
+         "void JacTR_SP_Vec(TJ JVS, TU UV, TO JTUV) {\n";
+    for (int i = 0; i < S.nvar; i++) o << "  JTUV[" << i << "] = " << md.jtuv_expr[i] << ";\n";
+    o << "}\n";
+    o << "template <class TJ, class TX>\nstatic __device__ __forceinline__ "
+         "void KppSolveTR(TJ JVS, TX X, TX XX) {\n";
+    for (auto& st : solve_tr_stmts(S)) o << st;
+    o << "}\n";
   }
   o << "#undef ARR_abc\n#undef ARR_ab\n#undef ARR_ac\n#undef EP2\n#undef EP3\n#undef FALL\n";
   o << "};  // struct " << ns << "\n";
@@ -2648,12 +3155,77 @@ void emit_stats(Model& md, const std::string& path) {
   o << "]\n}\n";
 }
 
+// KPP's C output files for host code that includes them
+// (src/code_c.c:55-540): <root>_Parameters.h with the sizes and species
+// indices of GenerateParamHeader (src/gen.c:2627-2700; C indices are
+// 0-based, src/gen.c:4116-4121) as `#define NAME value` lines, and
+// <root>_JacobianSP.c with the LU sparsity arrays of
+// GenerateJacobianSparseData (src/gen.c:528-580, row-compressed LU pattern,
+// 0-based).  Constants of features kppgen does not implement (families,
+// #LOOKAT/#MONITOR/#CHECK lists, Hessians, stoichiometric matrices) are not
+// written, so code that needs them fails to compile instead of using a
+// wrong value.
+void emit_kpp_c_files(Model& md, const std::string& base, const std::string& root) {
+  auto& S = md.S;
+  auto& m = *md.m;
+  {
+    std::ofstream o(base + "_Parameters.h");
+    std::string guard = upper(root) + "_PARAMETERS_H_INCLUDED";
+    o << "/* " << root << "_Parameters.h: sizes and species indices of mechanism " << root
+      << ", in the layout of KPP's C output (written by kppgen) */\n";
+    o << "#ifndef " << guard << "\n#define " << guard << "\n\n";
+    auto def = [&](const std::string& n, long v, const char* what) {
+      char b[160];
+      snprintf(b, sizeof b, "#define %-20s %-10ld /* %s */\n", n.c_str(), v, what);
+      o << b;
+    };
+    def("NSPEC", std::max(S.nspc, 1), "Number of chemical species");
+    def("NVAR", std::max(S.nvar, 1), "Number of variable species");
+    def("NFAM", std::max((int)m.families.size(), 1), "Number of prod/loss families");
+    def("NFIX", std::max(S.nfix, 1), "Number of fixed species");
+    def("NREACT", std::max(S.nreact, 1), "Number of reactions");
+    def("NVARST", 0, "Index of the first variable species in C");
+    def("NFIXST", S.nvar, "Index of the first fixed species in C");
+    def("NONZERO", std::max(S.nonzero, 1), "Nonzero entries of the Jacobian");
+    def("LU_NONZERO", std::max(S.lu_nonzero, 1), "Nonzero entries of its LU factors");
+    def("CNVAR", S.nvar + 1, "NVAR+1: length of the row-compressed arrays");
+    o << "\n/* Index declaration for variable species in C and VAR: VAR(ind_spc) = C(ind_spc) */\n";
+    for (int k = 0; k < S.nvar; k++) def("ind_" + m.spc(m.code[k]).name, k, "variable");
+    o << "\n/* Index declaration for fixed species in C: C(ind_spc) */\n";
+    for (int k = S.nvar; k < S.nspc; k++) def("ind_" + m.spc(m.code[k]).name, k, "fixed");
+    o << "\n/* Index declaration for fixed species in FIX: FIX(indf_spc) = C(NVAR+indf_spc) */\n";
+    for (int k = S.nvar; k < S.nspc; k++)
+      def("indf_" + m.spc(m.code[k]).name, k - S.nvar, "fixed, in FIX");
+    o << "\n#endif\n";
+  }
+  {
+    std::ofstream o(base + "_JacobianSP.c");
+    o << "/* " << root << "_JacobianSP.c: row-compressed sparsity pattern of the LU factors of "
+      << "the Jacobian of mechanism " << root << " (0-based), in the layout of KPP's C output "
+      << "(written by kppgen) */\n\n";
+    auto arr = [&](const char* name, const std::vector<int>& v, size_t n) {
+      o << "  int  " << name << "[] = {\n     ";
+      for (size_t i = 0; i < n; i++) {
+        char b[16];
+        snprintf(b, sizeof b, "%3d", v[i]);
+        o << b << (i + 1 < n ? "," : "");
+        if ((i + 1) % 12 == 0 && i + 1 < n) o << "\n     ";
+      }
+      o << " };\n\n";
+    };
+    arr("LU_IROW", S.lu_irow, (size_t)S.lu_nonzero);
+    arr("LU_ICOL", S.lu_icol, (size_t)S.lu_nonzero);
+    arr("LU_CROW", S.lu_crow, (size_t)S.nvar + 1);
+    arr("LU_DIAG", S.lu_diag, (size_t)S.nvar + 1);
+  }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   std::string in, out_dir = ".", root, kpp_home, env;
   std::vector<std::string> f90_modules;
-  bool do_reorder = true, rates_f90 = false;
+  bool do_reorder = true, rates_f90 = false, fun_split = false;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     auto next = [&]() -> std::string {
@@ -2731,12 +3303,16 @@ int main(int argc, char** argv) {
       else if (v.rfind("ON", 0) == 0) do_reorder = true;
       else die("#REORDER " + v + ": expected ON or OFF");
     } else if (c == "JACOBIAN") param(c, s.body, "SPARSE_LU_ROW");
-    else if (c == "FUNCTION") param(c, s.body, "AGGREGATE");
+    else if (c == "FUNCTION") {
+      std::string v = upper(trim(s.body));
+      while (!v.empty() && v.back() == ';') v = trim(v.substr(0, v.size() - 1));
+      if (v == "SPLIT") fun_split = true;
+      else if (v != "AGGREGATE") die("#FUNCTION " + v + ": expected AGGREGATE or SPLIT");
+    }
     else if (c == "DOUBLE") param(c, s.body, "ON");
     else if (c == "FLUX") param(c, s.body, "OFF");
     else if (c == "AUTOREDUCE") param(c, s.body, "OFF");
-    else if (c == "FAMILIES")
-      die("#FAMILIES is not supported: family species change the model functions");
+    else if (c == "FAMILIES") declare_families(m, s.body);
     else if (!kNoEffect.count(c)) die("unknown or unsupported KPP command #" + c);
   }
   // inline code that enters the rate constants (src/gen.c Update_RCONST):
@@ -2752,8 +3328,12 @@ int main(int argc, char** argv) {
   if (m.rates.empty()) die("no equations");
   Model md;
   md.m = &m;
+  md.fun_split = fun_split;
   md.S = reorder(m, do_reorder);
   md.build();
+  // the adjoint's model functions, for mechanisms whose Hessian is modest
+  md.adjoint = md.S.nvar <= 300;
+  if (md.adjoint) md.build_adjoint();
   f90::Program prog;
   if (rates_f90) {
     // inputs of the rate code: the per-cell values of the C ABI
@@ -2777,6 +3357,7 @@ int main(int argc, char** argv) {
   emit_c_model(md, base + "_model.h");
   emit_hip(md, base + "_hip.h");
   emit_stats(md, base + "_stats.json");
+  emit_kpp_c_files(md, base, root);
   printf("kppgen: %s NVAR=%d NFIX=%d NREACT=%d NONZERO=%d LU_NONZERO=%d\n", root.c_str(),
          md.S.nvar, md.S.nfix, md.S.nreact, md.S.nonzero, md.S.lu_nonzero);
   return 0;

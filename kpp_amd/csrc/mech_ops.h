@@ -49,6 +49,14 @@ typedef hipError_t (*ComponentFn)(int which, int n, size_t ld, const double* a, 
                                   const double* c, const double* d, const double* e, double* out,
                                   int* iout, hipStream_t st);
 
+// The SDIRK integrator of a mechanism (its own translation unit), and in the
+// same form the Runge-Kutta integrator (rows' argument unused); rows = 0
+// where it is not built.
+struct SdirkOps {
+  int (*rows)(int S);
+  hipError_t (*launch)(const RosParams& p, int nblocks, hipStream_t st);
+};
+
 // The mixed-precision integrator of a mechanism (its own translation unit):
 // FP32 LU and triangular solves with one FP64 refinement sweep per solve.
 struct MixedOps {

@@ -166,6 +166,11 @@ template <class M>
 KPPHIP_MODEL_CALL void call_fun(dptr V, dptr F, dptr R, dptr out) {
   M::Fun(Acc{V}, Acc{F}, Acc{R}, Acc{out});
 }
+// Fun_SPLIT of a mechanism compiled with #FUNCTION SPLIT (the F90 flavour)
+template <class M>
+KPPHIP_MODEL_CALL void call_fun_split(dptr V, dptr F, dptr R, dptr out) {
+  M::FunSplit(Acc{V}, Acc{F}, Acc{R}, Acc{out});
+}
 template <class M>
 KPPHIP_MODEL_CALL int call_jac_ghimj(dptr V, dptr F, dptr R, double ghinv, dptr G) {
   return M::JacGhimj(Acc{V}, Acc{F}, Acc{R}, ghinv, Acc{G});
@@ -214,6 +219,32 @@ template <class M>
 KPPHIP_MODEL_CALL void call_rates_fixed(double TEMP, double CFACTOR, dptr R) {
   M::rates_fixed(TEMP, CFACTOR, Acc{R});
 }
+// the integrators that keep the Jacobian between factorisations (SDIRK,
+// Runge-Kutta) and the complex LU of the Runge-Kutta integrator
+template <class M>
+KPPHIP_MODEL_CALL void call_jac_sp(dptr V, dptr F, dptr R, dptr J) {
+  M::Jac_SP(Acc{V}, Acc{F}, Acc{R}, Acc{J});
+}
+template <class M>
+KPPHIP_MODEL_CALL void call_jac_to_ghimj(dptr J, double ghinv, dptr G) {
+  M::JacToGhimj(Acc{J}, ghinv, Acc{G});
+}
+template <class M>
+KPPHIP_MODEL_CALL void call_diag_fill(dptr G, double v) {
+  M::DiagFill(Acc{G}, v);
+}
+template <class M>
+KPPHIP_MODEL_CALL int call_decomp_cmplx(dptr GR, dptr GI) {
+  return M::KppDecompCmplxR(Acc{GR}, Acc{GI});
+}
+template <class M>
+KPPHIP_MODEL_CALL void call_solve_cmplx(dptr GR, dptr GI, dptr XR, dptr XI) {
+  M::KppSolveCmplxR(Acc{GR}, Acc{GI}, Acc{XR}, Acc{XI});
+}
+template <class M>
+KPPHIP_MODEL_CALL int call_decomp_checked_g(dptr G) {
+  return M::KppDecompChecked(Acc{G});
+}
 
 // Storage of a lane's working set.
 //
@@ -230,6 +261,9 @@ struct TileMem {
   __device__ __forceinline__ A at(int row) const { return Acc{(gdouble*)(base + row * TILE)}; }
   __device__ __forceinline__ static void fun(A V, A F, A R, A out) {
     call_fun<M>(V.b, F.b, R.b, out.b);
+  }
+  __device__ __forceinline__ static void fun_split(A V, A F, A R, A out) {
+    call_fun_split<M>(V.b, F.b, R.b, out.b);
   }
   __device__ __forceinline__ static int jac_ghimj(A V, A F, A R, double ghinv, A G) {
     return call_jac_ghimj<M>(V.b, F.b, R.b, ghinv, G.b);
@@ -276,6 +310,9 @@ struct RegMem {
   mutable double a[ROWS];
   __device__ __forceinline__ A at(int row) const { return RegAcc{a + row}; }
   __device__ __forceinline__ static void fun(A V, A F, A R, A out) { M::Fun(V, F, R, out); }
+  __device__ __forceinline__ static void fun_split(A V, A F, A R, A out) {
+    M::FunSplit(V, F, R, out);
+  }
   __device__ __forceinline__ static int jac_ghimj(A V, A F, A R, double ghinv, A G) {
     return M::JacGhimj(V, F, R, ghinv, G);
   }
@@ -332,6 +369,8 @@ struct CellState {
   int Nfun, Njac, Nstp, Nacc, Nrej, Ndec, Nsol, Nsng;
   int Nfb;   // attempts that fell back from the FP32 to the FP64 factorisation
   int Nrun;  // rejected attempts since the last accepted one
+  double Tacc, Hnew;  // f90: T and predicted step after the last accepted step
+  double FacOld, ErrOld;  // the H211b filter's memory of the previous attempt
 };
 
 template <class M, int S, bool MIX = false, class Mem = TileMem<M>>
@@ -348,6 +387,19 @@ struct Cell {
   }
   __device__ __forceinline__ int lane_of() const { return (int)(threadIdx.x & (TILE - 1)); }
 
+  // FunTemplate's model call: Fun, or for the F90 flavour of a mechanism
+  // compiled with #FUNCTION SPLIT, Fun_SPLIT (KPP_FUN_OR_FUN_SPLIT,
+  // src/code.c:279-280, 361-362; the C integrators always call Fun)
+  __device__ __forceinline__ static void fcn(const RosParams& p, A V, A F, A R, A out) {
+    if constexpr (M::FUN_SPLIT) {
+      if (p.f90) {
+        Mem::fun_split(V, F, R, out);
+        return;
+      }
+    }
+    Mem::fun(V, F, R, out);
+  }
+
   // Rate constants at time T (int/rosenbrock.c:1257-1296 call Update_SUN and
   // Update_RCONST before every Fun/Jac).  Rates free of SUN and TIME were
   // computed once per cell with the same inputs, so not recomputing them
@@ -357,7 +409,7 @@ struct Cell {
   // TIME can have changed; `accepted` marks the first call after a new state.
   __device__ __forceinline__ void rates_at(const RosParams& p, const CellState& s, double T,
                                            bool accepted) const {
-    if (p.rates_given) return;
+    if (p.rates_given || p.rates_frozen) return;
     const bool tod = p.sun_mode == KPP_SUN_TOD;
     const double sun = tod ? update_sun(T) : s.SUN;
     if (tod || M::RATES_USE_TIME) Mem::rates_sun(s.TEMP, s.CFACTOR, sun, T, at(L::RCT));
@@ -388,12 +440,40 @@ struct Cell {
       if (p.sun_mode == KPP_SUN_FIXED && !M::RATES_USE_TIME)
         Mem::rates_sun(s.TEMP, s.CFACTOR, s.SUN, 0.0, R);
     }
+    s.Tstart = p.tstart ? p.tstart[cell] : p.tstart0;
+    if (p.rates_frozen && !p.rates_given) {
+      // every rate class once, at the cell's entry time and state
+      if (M::RATES_USE_TIME) Mem::rates_sun(s.TEMP, s.CFACTOR, s.SUN, s.Tstart, R);
+      if (M::RATES_USE_STATE) Mem::rates_state(s.TEMP, s.CFACTOR, s.SUN, s.Tstart, Y, F, R);
+    }
     s.Nfun = p.stat0[0]; s.Njac = p.stat0[1]; s.Nstp = p.stat0[2]; s.Nacc = p.stat0[3];
     s.Nrej = p.stat0[4]; s.Ndec = p.stat0[5]; s.Nsol = p.stat0[6]; s.Nsng = p.stat0[7];
     s.Nfb = 0;
     s.Nrun = 0;
-    s.Tstart = p.tstart ? p.tstart[cell] : p.tstart0;
     s.Tend = p.tend ? p.tend[cell] : p.tend0;
+    s.Tacc = 0.0;
+    s.Hnew = 0.0;
+    s.FacOld = 1.0;
+    s.ErrOld = 1.0;
+    if (p.f90) {
+      // int/rosenbrock.f90: Hmax, Hstart (:364-385) and the first step of
+      // ros_Integrator (:540-552), with a true ABS and the sign of direction
+      const double span = dabs(s.Tend - s.Tstart);
+      s.Hmax = (p.Hmax_user == 0.0) ? span : dmin(dabs(p.Hmax_user), span);
+      const double hs_user = p.hstart ? p.hstart[cell] : p.Hstart_user;
+      const double Hstart =
+          (hs_user == 0.0) ? dmax(p.Hmin, KPP_DELTAMIN_F90) : dmin(dabs(hs_user), span);
+      s.T = s.Tstart;
+      s.Direction = (s.Tend >= s.Tstart) ? +1 : -1;
+      double H = dmin(dmax(dabs(p.Hmin), dabs(Hstart)), dabs(s.Hmax));
+      if (dabs(H) <= 10.0 * p.Roundoff) H = KPP_DELTAMIN_F90;
+      s.H = (double)s.Direction * H;
+      s.Hexit = 0.0;
+      s.RejectLastH = false;
+      s.RejectMoreH = false;
+      s.newstep = true;
+      return;
+    }
     const double span = abs_diff_macro(s.Tend, s.Tstart);
     s.Hmax = (p.Hmax_user == 0.0) ? span : dmin(dabs(p.Hmax_user), span);
     const double hs_user = p.hstart ? p.hstart[cell] : p.Hstart_user;
@@ -467,8 +547,16 @@ struct Cell {
       q[6 * ld + cell] = s.Nsol; q[7 * ld + cell] = s.Nsng;
     }
     if (p.rstat) {
-      p.rstat[cell] = Texit;
-      p.rstat[ld + cell] = s.Hexit;
+      if (p.f90) {
+        // RSTATUS(1:3) of int/rosenbrock.f90: T, H and Hnew of the last
+        // accepted step (zero if none)
+        p.rstat[cell] = s.Tacc;
+        p.rstat[ld + cell] = s.Hexit;
+        p.rstat[2 * ld + cell] = s.Hnew;
+      } else {
+        p.rstat[cell] = Texit;
+        p.rstat[ld + cell] = s.Hexit;
+      }
     }
     if (p.fallbacks) p.fallbacks[cell] = s.Nfb;
   }
@@ -482,6 +570,7 @@ struct Cell {
     A Y = at(L::Y), F = at(L::FIX), R = at(L::RCT), Fcn0 = at(L::FCN0), Fcn = at(L::FCN),
       dFdT = at(L::DFDT), Ynew = at(L::YNEW), K0 = at(L::K), G = at(L::G);
     const bool skip_dfdt = p.dfdt_zero;
+    const bool f90 = p.f90 != 0;
     const double Dir = (double)s.Direction;
 
     if (s.newstep) {
@@ -491,11 +580,15 @@ struct Cell {
       if (!more) { finish(p, s, 1, s.T); return 1; }
       if (s.Nstp > p.max_steps) { finish(p, s, -6, s.T); return -6; }
       if (((s.T + 0.1 * s.H) == s.T) || (s.H <= Roundoff)) { finish(p, s, -7, s.T); return -7; }
-      s.Hexit = s.H;
-      s.H = dmin(s.H, abs_diff_macro(s.Tend, s.T));
+      if (p.f90) {
+        s.H = dmin(s.H, dabs(s.Tend - s.T));
+      } else {
+        s.Hexit = s.H;
+        s.H = dmin(s.H, abs_diff_macro(s.Tend, s.T));
+      }
       // Fcn0 = F(T, Y)
       rates_at(p, s, s.T, true);
-      Mem::fun(Y, F, R, Fcn0);
+      fcn(p, Y, F, R, Fcn0);
       s.Nfun++;
       // dF/dT by forward difference (int/rosenbrock.c:663-682)
       if (!p.autonomous) {
@@ -503,7 +596,7 @@ struct Cell {
         if (!skip_dfdt) {
           const double Delta = sqrt(Roundoff) * dmax(KPP_DELTAMIN, dabs(s.T));
           rates_at(p, s, s.T + Delta, false);
-          Mem::fun(Y, F, R, dFdT);
+          fcn(p, Y, F, R, dFdT);
           const double inv = 1.0 / Delta;
           vloop<U, NVAR>([&](int i) {
             const double d = dFdT[i] + (-1.0) * Fcn0[i];
@@ -562,13 +655,13 @@ struct Cell {
 #pragma unroll
           for (int j = 1; j <= istage - 1; j++) {
             const double a = m.A[(istage - 1) * (istage - 2) / 2 + j - 1];
-            if (a != 0.0) y = y + a * K0[(j - 1) * NVAR + i];
+            if (a != 0.0 || f90) y = y + a * K0[(j - 1) * NVAR + i];
           }
           Ynew[i] = y;
         });
         const double Tau = s.T + m.Alpha[istage - 1] * Dir * H;
         rates_at(p, s, Tau, false);
-        Mem::fun(Ynew, F, R, Fcn);
+        fcn(p, Ynew, F, R, Fcn);
         s.Nfun++;
       }
       // the row holding this stage's function values (tiles; in registers the
@@ -580,7 +673,10 @@ struct Cell {
         HC[j - 1] = m.C[(istage - 1) * (istage - 2) / 2 + j - 1] / (Dir * H);
       const double HG = Dir * H * m.Gamma[istage - 1];
       const bool use_g = (!p.autonomous) && (m.Gamma[istage - 1] != 0.0) && !skip_dfdt &&
-                         (HG != 0.0);
+                         (HG != 0.0 || f90);
+      // F90 adds HG*dFdT whenever Gamma_i /= 0; where dF/dT is exactly +0.0
+      // (not evaluated) that is HG*(+0.0)
+      const bool zero_g = f90 && (!p.autonomous) && (m.Gamma[istage - 1] != 0.0) && skip_dfdt;
       vloop<U, NVAR>([&](int i) {
         double k;
         if constexpr (U) {
@@ -595,8 +691,9 @@ struct Cell {
         }
 #pragma unroll
         for (int j = 1; j <= istage - 1; j++)
-          if (HC[j - 1] != 0.0) k = k + HC[j - 1] * K0[(j - 1) * NVAR + i];
+          if (HC[j - 1] != 0.0 || f90) k = k + HC[j - 1] * K0[(j - 1) * NVAR + i];
         if (use_g) k = k + HG * dFdT[i];
+        if (zero_g) k = k + HG * 0.0;
         Ki[i] = k;
       });
       if constexpr (MIX) {
@@ -650,28 +747,60 @@ struct Cell {
 #pragma unroll
       for (int j = 1; j <= S; j++) {
         const double kj = K0[(j - 1) * NVAR + i];
-        if (m.M[j - 1] != 0.0) yn = yn + m.M[j - 1] * kj;
-        if (m.E[j - 1] != 0.0) ye = ye + m.E[j - 1] * kj;
+        if (m.M[j - 1] != 0.0 || f90) yn = yn + m.M[j - 1] * kj;
+        if (m.E[j - 1] != 0.0 || f90) ye = ye + m.E[j - 1] * kj;
       }
       Ynew[i] = yn;
       const double Ymax = dmax(dabs(y), dabs(yn));
       const double Scale =
           p.vector_tol ? (p.atol[i] + p.rtol[i] * Ymax) : (p.atol0 + p.rtol0 * Ymax);
-      Err = Err + (ye * ye) / (Scale * Scale);
+      if (!f90) {
+        Err = Err + (ye * ye) / (Scale * Scale);
+      } else if (!p.np_w || p.np_w[i] != 0.0) {
+        // int/rosenbrock.f90:718-728: (Yerr/Scale)**2 over non-passive species
+        const double t = ye / Scale;
+        Err = Err + t * t;
+      }
     });
-    Err = sqrt(Err / (double)NVAR);
-    // step size control (:594-617)
-    const double Fac = dmin(p.FacMax, dmax(p.FacMin, p.FacSafe / pow(Err, 1.0 / m.ELO)));
+    Err = sqrt(Err / (double)(f90 ? p.np_count : NVAR));
+    if (f90) {
+      // ros_ErrorNorm's floor MAX(Err, 1e-10); a NaN estimate ends the cell
+      // (the floor would turn it into an accepted step)
+      if (Err != Err) {
+        finish(p, s, KPP_IERR_NONFINITE, s.T);
+        return KPP_IERR_NONFINITE;
+      }
+      Err = dmax(Err, 1.0e-10);
+    }
+    // step size control (:594-617), or the H211b filter of
+    // int/rosenbrock_h211b_qssa.f90:882-892 (no FacMin/FacMax bounds)
+    double Fac;
+    if (p.h211b) {
+      const double e = 1.0 / (p.bb * p.kk);
+      Fac = pow(1.0 / Err, e) * pow(1.0 / s.ErrOld, e) * pow(s.FacOld, -1.0 / p.bb);
+      s.FacOld = Fac;
+      s.ErrOld = Err;
+    } else {
+      Fac = dmin(p.FacMax, dmax(p.FacMin, p.FacSafe / pow(Err, 1.0 / m.ELO)));
+    }
     double Hnew = H * Fac;
     s.Nstp++;
     if ((Err <= 1.0) || (H <= p.Hmin)) {
       // step log: size of the first accepted step of this call
       if (p.hfirst && s.Nacc == p.stat0[3]) p.hfirst[s.cell] = H;
       s.Nacc++;
-      vloop<U, NVAR>([&](int i) { Y[i] = Ynew[i]; });
+      if (p.nonneg)
+        vloop<U, NVAR>([&](int i) { Y[i] = dmax(Ynew[i], 0.0); });  // MAX(Ynew, ZERO)
+      else
+        vloop<U, NVAR>([&](int i) { Y[i] = Ynew[i]; });
       s.T += Dir * H;
       Hnew = dmax(p.Hmin, dmin(Hnew, s.Hmax));
       if (s.RejectLastH) Hnew = dmin(Hnew, H);
+      if (f90) {
+        s.Hexit = H;
+        s.Hnew = Hnew;
+        s.Tacc = s.T;
+      }
       s.RejectLastH = false;
       s.RejectMoreH = false;
       s.H = Hnew;

@@ -113,6 +113,27 @@ def library() -> ctypes.CDLL:
     L.kpp_get_state.argtypes = [vp, vp, vp, vp, vp]
     L.kpp_ctx_set_tol.argtypes = [vp, vp, vp, ci]
     L.INTEGRATE_BATCH.argtypes = [vp, cd, cd]
+    L.kpp_integrate_f90_dev.argtypes = [cs, ctypes.POINTER(_Batch), vp, vp, vp, vp, vp,
+                                        ctypes.POINTER(_Launch), vp, ctypes.c_size_t, vp]
+    L.kpp_workspace_bytes_f90.restype = ctypes.c_size_t
+    L.kpp_workspace_bytes_f90.argtypes = [cs, ci, ctypes.POINTER(_Launch), ci]
+    L.kpp_check_params_f90.argtypes = [vp, vp, vp, vp, ci]
+    L.INTEGRATE_F90_BATCH.argtypes = [vp, cd, cd, vp, vp, vp, vp, vp]
+    L.kpp_ctx_set_passive.argtypes = [vp, vp]
+    L.kpp_integrate_h211b_dev.argtypes = [cs, ctypes.POINTER(_Batch), vp, vp, vp, vp,
+                                          ctypes.POINTER(_Launch), vp, ctypes.c_size_t, vp]
+    L.kpp_workspace_bytes_sdirk.restype = ctypes.c_size_t
+    L.kpp_workspace_bytes_sdirk.argtypes = [cs, ci, ci]
+    L.kpp_sdirk_dev.argtypes = [cs, ctypes.POINTER(_Batch), vp, vp, vp, vp, vp, ctypes.c_size_t,
+                                vp]
+    L.kpp_workspace_bytes_rk.restype = ctypes.c_size_t
+    L.kpp_workspace_bytes_rk.argtypes = [cs, ci]
+    L.kpp_rk_dev.argtypes = [cs, ctypes.POINTER(_Batch), vp, vp, vp, vp, vp, ctypes.c_size_t,
+                             vp]
+    L.kpp_workspace_bytes_adj.restype = ctypes.c_size_t
+    L.kpp_workspace_bytes_adj.argtypes = [cs, ci, vp, ci, ci]
+    L.kpp_rosenbrock_adj_dev.argtypes = [cs, ctypes.POINTER(_Batch), vp, vp, vp, vp, ci, vp, ci,
+                                         ci, vp, ctypes.c_size_t, vp]
     _lib = L
     return L
 
@@ -287,6 +308,191 @@ def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, R
     if code <= -100:
         raise KppError(f"kpp_rosenbrock_dev failed with {code}")
     return code, ierr, istat, rstat
+
+
+def integrate_f90(mech: str, C, temp, cfactor, *, tin, tout, atol, rtol, ICNTRL=None,
+                  RCNTRL=None, sun=None, passive=None, hstart=None, rconst=None, ierr=None,
+                  istat=None, rstat=None, variant=PLAIN, nblocks=0, lanes=0, workspace=None,
+                  stream=None, h211b=False):
+    """The F90 INTEGRATE (int/rosenbrock.f90:43-125) on device tensors, one
+    call per cell over [tin, tout] (per-cell tensors or floats).
+
+    ICNTRL/RCNTRL: the caller's ICNTRL_U/RCNTRL_U (20 entries; None = absent),
+    merged as INTEGRATE merges them.  atol/rtol: NVAR entries (ICNTRL(2) = 0)
+    or 1.  passive: NVAR flags (1 = left out of the error norm) or None.
+    Returns (code, ierr, istat [8, n] = ISTATUS(1:8), rstat [3, n] = RSTATUS(1:3)).
+    h211b=True: the INTEGRATE of int/rosenbrock_h211b_qssa.f90 (kpp_integrate_h211b_dev;
+    H211b control with RCNTRL(15), RCNTRL(16); no passive species).
+    """
+    import torch
+    L = library()
+    n = int(C.shape[1])
+    dev = C.device
+    if C.dtype != torch.float64 or not C.is_contiguous():
+        raise KppError("C must be a contiguous float64 tensor [NSPEC, ncells]")
+    if ierr is None:
+        ierr = torch.empty(n, dtype=torch.int32, device=dev)
+    if istat is None:
+        istat = torch.empty((8, n), dtype=torch.int32, device=dev)
+    if rstat is None:
+        rstat = torch.empty((3, n), dtype=torch.float64, device=dev)
+    b = _Batch()
+    b.ncells = n
+    b.ld = n
+    b.C = C.data_ptr()
+    b.temp = temp.data_ptr()
+    b.cfactor = cfactor.data_ptr()
+    b.sun = sun.data_ptr() if sun is not None else None
+    if torch.is_tensor(tin):
+        b.tstart = tin.data_ptr()
+    else:
+        b.tstart0 = float(tin)
+    if torch.is_tensor(tout):
+        b.tend = tout.data_ptr()
+    else:
+        b.tend0 = float(tout)
+    b.hstart = hstart.data_ptr() if hstart is not None else None
+    b.rconst = rconst.data_ptr() if rconst is not None else None
+    b.ierr = ierr.data_ptr()
+    b.istat = istat.data_ptr()
+    b.rstat = rstat.data_ptr()
+    ic = None if ICNTRL is None else np.ascontiguousarray(ICNTRL, np.int32)
+    rc = None if RCNTRL is None else np.ascontiguousarray(RCNTRL, np.float64)
+    pas = None if passive is None else np.ascontiguousarray(passive, np.int32)
+    atol = np.ascontiguousarray(np.atleast_1d(atol), np.float64)
+    rtol = np.ascontiguousarray(np.atleast_1d(rtol), np.float64)
+    l = _Launch(variant, nblocks, 0, lanes, 0, 0)
+    icm = 0 if ic is None or ic[2] == 0 else int(ic[2])
+    nbytes = L.kpp_workspace_bytes_f90(mech.encode(), icm, ctypes.byref(l), n)
+    if workspace is None:
+        workspace = _default_ws.setdefault(str(dev), Workspace(dev))
+    ws = workspace.get(max(nbytes, 256))
+    ptr = lambda a: None if a is None else a.ctypes.data
+    if h211b:
+        if pas is not None:
+            raise KppError("the H211b integrator has no passive species")
+        code = L.kpp_integrate_h211b_dev(mech.encode(), ctypes.byref(b), ptr(ic), ptr(rc),
+                                         atol.ctypes.data, rtol.ctypes.data, ctypes.byref(l),
+                                         ctypes.c_void_p(ws.data_ptr()),
+                                         ctypes.c_size_t(ws.numel()), _stream_ptr(stream))
+    else:
+        code = L.kpp_integrate_f90_dev(mech.encode(), ctypes.byref(b), ptr(ic), ptr(rc),
+                                       atol.ctypes.data, rtol.ctypes.data, ptr(pas),
+                                       ctypes.byref(l), ctypes.c_void_p(ws.data_ptr()),
+                                       ctypes.c_size_t(ws.numel()), _stream_ptr(stream))
+    if code <= -100:
+        raise KppError(f"kpp_integrate_f90_dev failed with {code}")
+    return code, ierr, istat, rstat
+
+
+def _implicit(which, mech, C, temp, cfactor, tstart, tend, atol, rtol, ICNTRL, RCNTRL, sun,
+              sun_mode, hstart, rconst, ierr, istat, rstat, workspace, stream, adj=None):
+    """kpp_sdirk_dev / kpp_rk_dev / kpp_rosenbrock_adj_dev on device tensors
+    (the shared batch set-up); adj = (nadj, Lambda, nchk, first_step)"""
+    import torch
+    L = library()
+    n = int(C.shape[1])
+    dev = C.device
+    if C.dtype != torch.float64 or not C.is_contiguous():
+        raise KppError("C must be a contiguous float64 tensor [NSPEC, ncells]")
+    ierr = torch.empty(n, dtype=torch.int32, device=dev) if ierr is None else ierr
+    istat = torch.empty((8, n), dtype=torch.int32, device=dev) if istat is None else istat
+    rstat = torch.empty((3, n), dtype=torch.float64, device=dev) if rstat is None else rstat
+    b = _Batch()
+    b.ncells = n
+    b.ld = n
+    b.C = C.data_ptr()
+    b.temp = temp.data_ptr()
+    b.cfactor = cfactor.data_ptr()
+    b.sun = sun.data_ptr() if sun is not None else None
+    b.sun_mode = sun_mode
+    if torch.is_tensor(tstart):
+        b.tstart = tstart.data_ptr()
+    else:
+        b.tstart0 = float(tstart)
+    if torch.is_tensor(tend):
+        b.tend = tend.data_ptr()
+    else:
+        b.tend0 = float(tend)
+    b.hstart = hstart.data_ptr() if hstart is not None else None
+    b.rconst = rconst.data_ptr() if rconst is not None else None
+    b.ierr = ierr.data_ptr()
+    b.istat = istat.data_ptr()
+    b.rstat = rstat.data_ptr()
+    ic = np.zeros(20, np.int32) if ICNTRL is None else np.ascontiguousarray(ICNTRL, np.int32)
+    rc = np.zeros(20) if RCNTRL is None else np.ascontiguousarray(RCNTRL, np.float64)
+    atol = np.ascontiguousarray(np.atleast_1d(atol), np.float64)
+    rtol = np.ascontiguousarray(np.atleast_1d(rtol), np.float64)
+    if which == "sdirk":
+        nbytes = L.kpp_workspace_bytes_sdirk(mech.encode(), int(ic[2]), n)
+        fn = L.kpp_sdirk_dev
+    elif which == "rk":
+        nbytes = L.kpp_workspace_bytes_rk(mech.encode(), n)
+        fn = L.kpp_rk_dev
+    else:
+        nadj, lam, nchk, first = adj
+        nbytes = L.kpp_workspace_bytes_adj(mech.encode(), n, ic.ctypes.data, nadj, nchk)
+        fn = L.kpp_rosenbrock_adj_dev
+        if nbytes == 0 and int(ic[2]) in (0, 1, 2, 3, 4, 5, 7):
+            raise KppError(f"the adjoint integrator is not built for {mech}")
+    if workspace is None:
+        workspace = _default_ws.setdefault(str(dev), Workspace(dev))
+    ws = workspace.get(max(nbytes, 256))
+    head = (mech.encode(), ctypes.byref(b), ic.ctypes.data, rc.ctypes.data, atol.ctypes.data,
+            rtol.ctypes.data)
+    tail = (ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws.numel()), _stream_ptr(stream))
+    if which == "adj":
+        code = fn(*head, nadj, ctypes.c_void_p(lam.data_ptr() if lam is not None else 0), nchk,
+                  1 if first else 0, *tail)
+    else:
+        code = fn(*head, *tail)
+    if code <= -100:
+        raise KppError(f"{fn.__name__} failed with {code}")
+    return code, ierr, istat, rstat
+
+
+def sdirk(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, ICNTRL=None, RCNTRL=None,
+          sun=None, sun_mode=SUN_FIXED, hstart=None, rconst=None, ierr=None, istat=None,
+          rstat=None, workspace=None, stream=None):
+    """KPP's C SDIRK (int/sdirk.c) on device tensors, one lane per cell.
+    ICNTRL/RCNTRL: 20 entries as SDIRK takes them (None = zeros).  Returns
+    (code, ierr, istat [8, n] = ISTATUS[1..8], rstat [3, n] = RSTATUS[1..3])."""
+    return _implicit("sdirk", mech, C, temp, cfactor, tstart, tend, atol, rtol, ICNTRL, RCNTRL,
+                     sun, sun_mode, hstart, rconst, ierr, istat, rstat, workspace, stream)
+
+
+def runge_kutta(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, ICNTRL=None,
+                RCNTRL=None, sun=None, sun_mode=SUN_FIXED, hstart=None, rconst=None, ierr=None,
+                istat=None, rstat=None, workspace=None, stream=None):
+    """KPP's C fully implicit Runge-Kutta (int/runge_kutta.c) on device
+    tensors, one lane per cell.  ICNTRL/RCNTRL: 20 entries as RungeKutta
+    takes them (None = zeros: Radau-2A, classical error estimate,
+    Gustafsson's controller).  Returns (code, ierr, istat [8, n] =
+    ISTATUS[0..7], rstat [3, n] = RSTATUS[0..2])."""
+    return _implicit("rk", mech, C, temp, cfactor, tstart, tend, atol, rtol, ICNTRL, RCNTRL,
+                     sun, sun_mode, hstart, rconst, ierr, istat, rstat, workspace, stream)
+
+
+def rosenbrock_adj(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, Lambda,
+                   ICNTRL=None, RCNTRL=None, nchk=4096, first_step=False, sun=None,
+                   sun_mode=SUN_FIXED, rconst=None, ierr=None, istat=None, rstat=None,
+                   workspace=None, stream=None):
+    """KPP's C discrete adjoint Rosenbrock (int/rosenbrock_adj.c) on device
+    tensors, one lane per cell.  Lambda: float64 [NADJ*NVAR, ncells] (row
+    a*NVAR + i = Lambda[a][i]), dg/dy at tend on entry, at tstart on return;
+    nchk checkpoint slots per cell; first_step False keeps the reference's
+    backward loop (the first accepted step left out).  Returns (code, ierr,
+    istat [8, n] = ISTATUS[1..8], rstat [3, n] = RSTATUS[1..3])."""
+    import torch
+    nv = mechanism(mech).nvar
+    nadj = 0 if Lambda is None else int(Lambda.shape[0]) // nv
+    if Lambda is not None:
+        if (Lambda.dtype != torch.float64 or not Lambda.is_contiguous() or
+                Lambda.shape[0] != nadj * nv or Lambda.shape[1] != C.shape[1]):
+            raise KppError("Lambda must be a contiguous float64 tensor [NADJ*NVAR, ncells]")
+    return _implicit("adj", mech, C, temp, cfactor, tstart, tend, atol, rtol, ICNTRL, RCNTRL,
+                     sun, sun_mode, None, rconst, ierr, istat, rstat, workspace, stream,
+                     adj=(nadj, Lambda, int(nchk), bool(first_step)))
 
 
 def _comp(fn, *args):

@@ -99,18 +99,25 @@ def _scale_var(vals, nvar, seed, cells):
     return C
 
 
-def small_strato_tod(ncells=4096, seed=1, cells=None, rtol=1e-3, atol=1.0, method=solver.ROS3):
+def small_strato_tod(ncells=4096, seed=1, cells=None, rtol=1e-3, atol=1.0, method=solver.ROS3,
+                     mech="small_strato"):
     cells = np.arange(ncells, dtype=np.int64) if cells is None else np.asarray(cells)
-    vals, cf, st = mech_init("small_strato")
+    vals, cf, st = mech_init(mech)
     nvar = st["NVAR"]
     C = _scale_var(vals * cf, nvar, seed, cells)
     tod = 24.0 * uniform(seed, F_TOD, cells)
     tstart = tod * 3600.0
     n = len(cells)
-    return Batch("small_strato", method, C, np.full(n, 270.0), np.full(n, cf), np.zeros(n),
+    return Batch(mech, method, C, np.full(n, 270.0), np.full(n, cf), np.zeros(n),
                  solver.SUN_TOD, tstart, tstart + 3600.0, np.array([atol]), np.array([rtol]),
                  solver.default_ipar(method), np.zeros(20), cells,
-                 {"workload": "small_strato_tod"})
+                 {"workload": f"{mech}_tod"})
+
+
+def strato_fam_tod(ncells=4096, seed=1, cells=None, rtol=1e-3, atol=1.0, method=solver.ROS3):
+    """small_strato_tod's recipe on the #FAMILIES test mechanism
+    (kpp_amd/mechs/strato_fam.kpp); the family species POx, LOx start at 0."""
+    return small_strato_tod(ncells, seed, cells, rtol, atol, method, mech="strato_fam")
 
 
 def _saprc99(ncells, seed, cells, uneven, rtol, atol, method, mech="saprc99"):
@@ -189,6 +196,24 @@ def linear6(ncells=1, method=solver.RODAS3, rtol=1e-3, atol=1.0, temp=1.0e-3):
     return Batch("linear6", method, C, np.full(ncells, temp), np.full(ncells, cf), z.copy(),
                  solver.SUN_FIXED, z.copy(), z + 3600.0, np.array([atol]), np.array([rtol]),
                  solver.default_ipar(method), np.zeros(20), cells, {"workload": "linear6"})
+
+
+def carbon_box(ncells=4096, seed=11, cells=None, rtol=1e-3, atol=1.0, method=solver.RODAS3):
+    """KPP's carbon example (models/carbon.*): its #INITVALUES (CH4, CO, the
+    fixed OH and the dummy precursors) x 10^U(-1,1) per species and cell,
+    T ~ U(260, 310) K, three days from t = 0 as the example's driver runs."""
+    cells = np.arange(ncells, dtype=np.int64) if cells is None else np.asarray(cells)
+    vals, cf, st = mech_init("carbon")
+    nvar, nspec = st["NVAR"], len(vals)
+    C = np.repeat((vals * cf)[:, None], len(cells), axis=1)
+    for s in range(nspec):
+        C[s] *= 10.0 ** (2.0 * uniform(seed, F_SPC + s, cells) - 1.0)
+    n = len(cells)
+    T = 260.0 + 50.0 * uniform(seed, F_TEMP, cells)
+    tstart = np.zeros(n)
+    return Batch("carbon", method, C, T, np.full(n, cf), np.zeros(n), solver.SUN_FIXED, tstart,
+                 tstart + 3.0 * 24.0 * 3600.0, np.array([atol]), np.array([rtol]),
+                 solver.default_ipar(method), np.zeros(20), cells, {"workload": "carbon_box"})
 
 
 WORKLOADS = {

@@ -43,6 +43,9 @@
 #endif
 
 #include KPP_MODEL_HEADER
+#ifndef KPP_FUN_SPLIT
+#define KPP_FUN_SPLIT 0
+#endif
 
 #define MAX(a, b) (((a) >= (b)) ? (a) : (b))
 #define MIN(b, c) (((b) < (c)) ? (b) : (c))
@@ -72,6 +75,7 @@ typedef struct {
   double Hacc1; /* step size of the first accepted step of the call (step log), 0 if none */
   const double* Yacc; /* the accepted state (the integrator's Y): what rate code reading
                          concentrations sees, as KPP's global C */
+  int fun_split; /* F90 flavour of a #FUNCTION SPLIT mechanism: FunTemplate calls Fun_SPLIT */
 } ros_cell;
 
 typedef struct {
@@ -155,6 +159,14 @@ static void update_rates(ros_cell* c, double T) {
 }
 static void FunTemplate(ros_cell* c, double T, const double* Y, double* Ydot) {
   update_rates(c, T);
+#if KPP_FUN_SPLIT
+  /* KPP_FUN_OR_FUN_SPLIT of the F90 integrators (src/code.c:279-280, 361-362) */
+  if (c->fun_split) {
+    kpp_Fun_SPLIT(Y, c->FIX, c->RCONST, Ydot);
+    c->st.Nfun++;
+    return;
+  }
+#endif
   kpp_Fun(Y, c->FIX, c->RCONST, Ydot);
   c->st.Nfun++;
 }
@@ -575,6 +587,396 @@ static int Rosenbrock(ros_cell* c, double* Y, double Tstart, double Tend, const 
   return IERR;
 }
 
+/* ======================= F90 Rosenbrock (int/rosenbrock.f90) =======================
+ * KPP's Fortran 90 integrator is a separate implementation with its own
+ * method list, defaults and outputs:
+ *   INTEGRATE        :43-125   ICNTRL(15) = 5 by default; ICNTRL_U overrides
+ *                               where /= 0, RCNTRL_U where > 0
+ *   Rosenbrock       :128-446  ICNTRL(3) map (0,4 Rodas3; 1 Ros2; 2 Ros3;
+ *                               3 Ros4; 5 Rodas4; 6 Rang3; 7 Rodas3.1),
+ *                               200000 steps, Hstart = MAX(Hmin, 1e-5),
+ *                               -1..-5 checks
+ *   ros_Integrator   :492-686  H = MIN(MAX(|Hmin|,|Hstart|),|Hmax|) signed by
+ *                               the direction; array updates without the
+ *                               zero-coefficient skips of the C WAXPY;
+ *                               ICNTRL(16) clipping; RSTATUS(1:3) = T, H, Hnew
+ *                               of the last accepted step
+ *   ros_ErrorNorm    :690-748  non-passive species only, (Yerr/Scale)**2,
+ *                               MAX(Err, 1e-10)
+ *   Ros2 g = 1 + 1/SQRT(2); Rodas3 C(6) = -(8/3); Rang3 :1327-1394.
+ * Its FunTemplate/JacTemplate (:1405-1478) call Update_SUN/Update_RCONST as
+ * ICNTRL(15) selects (util/util.f90:133-159); the rate code of the supported
+ * mechanisms reads concentrations from the global C, i.e. the accepted state,
+ * as in the C path.  int/rosenbrock_h211b_qssa.f90 is the same integrator
+ * (INTEGRATE :56-138, ros_Integrator :612-935, the norm over all species
+ * :939-969) with the H211b step-size filter; its QSSA options are not
+ * restated.  Deviations, both shared with the device: a NaN error
+ * estimate ends the cell with -9 (gfortran's MAX would make it 1e-10 and
+ * accept the step), and more than MAX_REJECT rejections in a row with -10. */
+#define DeltaMinF90 1.0e-5
+#define F90MAX(a, b) (((a) >= (b)) ? (a) : (b))
+
+static void Ros2_f90(ros_method* m) {
+  Ros2(m);
+  double g = 1.0 + 1.0 / sqrt(2.0);
+  m->A[0] = 1.0 / g;
+  m->C[0] = (-2.0) / g;
+  m->M[0] = (3.0) / (2.0 * g);
+  m->M[1] = (1.0) / (2.0 * g);
+  m->E[0] = 1.0 / (2.0 * g);
+  m->E[1] = 1.0 / (2.0 * g);
+  m->Gamma[0] = g;
+  m->Gamma[1] = -g;
+}
+static void Rodas3_f90(ros_method* m) {
+  Rodas3(m);
+  m->C[5] = -(8.0 / 3.0);
+}
+static void Rang3(ros_method* m) {
+  /* Rang & Angermann, BIT 45 (2005) 761-787, Tables 4.1-4.2, as tabled in
+   * int/rosenbrock.f90:1348-1392 */
+  m->S = 4;
+  m->A[0] = 5.09052051067020; m->A[1] = 5.09052051067020; m->A[2] = 0.0;
+  m->A[3] = 4.97628111010787; m->A[4] = 2.77268164715849e-02; m->A[5] = 2.29428036027904e-01;
+  m->C[0] = -1.16790812312283e+01; m->C[1] = -1.64057326467367e+01;
+  m->C[2] = -2.77268164715850e-01; m->C[3] = -8.38103960500476;
+  m->C[4] = -8.48328409199343e-01; m->C[5] = 2.87009860433106e-01;
+  m->M[0] = 5.22582761233094; m->M[1] = -5.56971148154165e-01;
+  m->M[2] = 3.57979469353645e-01; m->M[3] = 1.72337398521064;
+  m->E[0] = -5.16845212784040; m->E[1] = -1.26351942603842;
+  m->E[2] = -1.11022302462516e-16; m->E[3] = 2.22044604925031e-16;
+  m->Alpha[0] = 0.0; m->Alpha[1] = 2.21878746765329; m->Alpha[2] = 2.21878746765329;
+  m->Alpha[3] = 1.55392337535788;
+  m->Gamma[0] = 4.35866521508459e-01; m->Gamma[1] = -1.78292094614483;
+  m->Gamma[2] = -2.46541900496934; m->Gamma[3] = -8.05529997906370e-01;
+  for (int i = 0; i < 4; i++) m->NewF[i] = 1;
+  m->ELO = 3.0;
+}
+/* the ICNTRL(3) map of int/rosenbrock.f90:321-340; 0 if unknown */
+static int method_f90(int code, ros_method* m) {
+  memset(m, 0, sizeof *m);
+  switch (code) {
+    case 0: case 4: Rodas3_f90(m); return 1;
+    case 1: Ros2_f90(m); return 1;
+    case 2: Ros3(m); return 1;
+    case 3: Ros4(m); return 1;
+    case 5: Rodas4(m); return 1;
+    case 6: Rang3(m); return 1;
+    case 7: Rodas3_1(m); return 1;
+  }
+  return 0;
+}
+int ros_ref_method_f90(int code, int* S, double* A, double* C, double* M, double* E,
+                       double* Alpha, double* Gamma, double* ELO, int* NewF) {
+  ros_method m;
+  if (!method_f90(code, &m)) return 0;
+  *S = m.S;
+  for (int i = 0; i < Smax * (Smax - 1) / 2; i++) { A[i] = m.A[i]; C[i] = m.C[i]; }
+  for (int i = 0; i < Smax; i++) {
+    M[i] = m.M[i]; E[i] = m.E[i]; Alpha[i] = m.Alpha[i]; Gamma[i] = m.Gamma[i];
+    NewF[i] = m.NewF[i];
+  }
+  *ELO = m.ELO;
+  return 1;
+}
+
+typedef struct {
+  const int* passive; /* [NVAR] 1 = passive (left out of the error norm), or NULL */
+  int np_count;       /* NonPassiveSpc_Count */
+  int nonneg;         /* ICNTRL(16) == 1 */
+  int h211b;          /* int/rosenbrock_h211b_qssa.f90 with bb, kk > 0: the H211b controller */
+  double bb, kk;      /* RCNTRL(15), RCNTRL(16) */
+} f90_opts;
+
+/* ros_ErrorNorm, int/rosenbrock.f90:690-748 (returns the norm before the
+ * MAX with 1e-10, which the caller applies) */
+static double ros_ErrorNorm_f90(const double* Y, const double* Ynew, const double* Yerr,
+                                const double* AbsTol, const double* RelTol, int VectorTol,
+                                const f90_opts* o) {
+  double Err = ZERO, Scale, Ymax, t;
+  for (int i = 0; i < KPP_NVAR; i++) {
+    if (o->passive && o->passive[i]) continue;
+    Ymax = F90MAX(fabs(Y[i]), fabs(Ynew[i]));
+    Scale = VectorTol ? (AbsTol[i] + RelTol[i] * Ymax) : (AbsTol[0] + RelTol[0] * Ymax);
+    t = Yerr[i] / Scale;
+    Err = Err + t * t;
+  }
+  return sqrt(Err / (double)o->np_count);
+}
+
+/* ros_Integrator, int/rosenbrock.f90:492-686; rstatus[0..2] = RSTATUS(1:3) */
+static int ros_Integrator_f90(ros_cell* c, double* Y, double Tstart, double Tend,
+                              const double* AbsTol, const double* RelTol, const ros_method* m,
+                              int Autonomous, int VectorTol, int Max_no_steps, double Roundoff,
+                              double Hmin, double Hmax, double Hstart, double FacMin,
+                              double FacMax, double FacRej, double FacSafe, const f90_opts* o,
+                              double* rstatus) {
+  double Ynew[KPP_NVAR], Fcn0[KPP_NVAR], Fcn[KPP_NVAR], dFdT[KPP_NVAR];
+  double Jac0[KPP_LU_NONZERO], Ghimj[KPP_LU_NONZERO];
+  double K[KPP_NVAR * Smax], Yerr[KPP_NVAR];
+  double H, T, Hnew, HC, HG, Fac, Tau, Err, Delta;
+  double FacOld = 1.0, ErrOld = 1.0; /* the H211b filter's memory */
+  int Direction, j, istage, i;
+  int RejectLastH, RejectMoreH;
+  int Nrun = 0;
+  const int N = KPP_NVAR, ros_S = m->S;
+
+  T = Tstart;
+  rstatus[1] = ZERO;
+  c->Yacc = Y;
+  H = MIN(F90MAX(fabs(Hmin), fabs(Hstart)), fabs(Hmax));
+  if (fabs(H) <= 10.0 * Roundoff) H = DeltaMinF90;
+  Direction = (Tend >= Tstart) ? +1 : -1;
+  H = Direction * H;
+  RejectLastH = 0;
+  RejectMoreH = 0;
+
+  while (((Direction > 0) && ((T - Tend) + Roundoff <= ZERO)) ||
+         ((Direction < 0) && ((Tend - T) + Roundoff <= ZERO))) {
+    if (c->st.Nstp > Max_no_steps) return -6;
+    if (((T + 0.1 * H) == T) || (H <= Roundoff)) return -7;
+    H = MIN(H, fabs(Tend - T));
+    FunTemplate(c, T, Y, Fcn0);
+    if (!Autonomous) {
+      /* ros_FunTimeDerivative :754-778 (DeltaMin 1e-6 there) */
+      Delta = sqrt(Roundoff) * F90MAX(DeltaMin, fabs(T));
+      FunTemplate(c, T + Delta, Y, dFdT);
+      for (i = 0; i < N; i++) dFdT[i] = dFdT[i] - Fcn0[i];
+      for (i = 0; i < N; i++) dFdT[i] = dFdT[i] * (ONE / Delta);
+    }
+    JacTemplate(c, T, Y, Jac0);
+    for (;;) {
+      if (ros_PrepareMatrix(c, &H, Direction, m->Gamma[0], Jac0, Ghimj)) return -8;
+      for (istage = 1; istage <= ros_S; istage++) {
+        double* Ki = &K[N * (istage - 1)];
+        if (istage == 1) {
+          for (i = 0; i < N; i++) Fcn[i] = Fcn0[i];
+        } else if (m->NewF[istage - 1]) {
+          for (i = 0; i < N; i++) Ynew[i] = Y[i];
+          for (j = 1; j <= istage - 1; j++) {
+            const double a = m->A[(istage - 1) * (istage - 2) / 2 + j - 1];
+            for (i = 0; i < N; i++) Ynew[i] = Ynew[i] + a * K[N * (j - 1) + i];
+          }
+          Tau = T + m->Alpha[istage - 1] * Direction * H;
+          FunTemplate(c, Tau, Ynew, Fcn);
+        }
+        for (i = 0; i < N; i++) Ki[i] = Fcn[i];
+        for (j = 1; j <= istage - 1; j++) {
+          HC = m->C[(istage - 1) * (istage - 2) / 2 + j - 1] / (Direction * H);
+          for (i = 0; i < N; i++) Ki[i] = Ki[i] + HC * K[N * (j - 1) + i];
+        }
+        if ((!Autonomous) && (m->Gamma[istage - 1] != ZERO)) {
+          HG = Direction * H * m->Gamma[istage - 1];
+          for (i = 0; i < N; i++) Ki[i] = Ki[i] + HG * dFdT[i];
+        }
+        kpp_KppSolve(Ghimj, Ki);
+        c->st.Nsol++;
+      }
+      for (i = 0; i < N; i++) Ynew[i] = Y[i];
+      for (j = 1; j <= ros_S; j++)
+        for (i = 0; i < N; i++) Ynew[i] = Ynew[i] + m->M[j - 1] * K[N * (j - 1) + i];
+      for (i = 0; i < N; i++) Yerr[i] = ZERO;
+      for (j = 1; j <= ros_S; j++)
+        for (i = 0; i < N; i++) Yerr[i] = Yerr[i] + m->E[j - 1] * K[N * (j - 1) + i];
+      Err = ros_ErrorNorm_f90(Y, Ynew, Yerr, AbsTol, RelTol, VectorTol, o);
+      if (Err != Err) return IERR_NONFINITE;
+      Err = F90MAX(Err, 1.0e-10);
+      if (o->h211b) {
+        /* H211b controller (int/rosenbrock_h211b_qssa.f90:882-892, after
+         * Soderlind 2003): no FacMin/FacMax bounds, memory of the previous
+         * attempt's error and factor */
+        Fac = pow(ONE / Err, 1.0 / (o->bb * o->kk)) * pow(ONE / ErrOld, 1.0 / (o->bb * o->kk)) *
+              pow(FacOld, -1.0 / o->bb);
+        FacOld = Fac;
+        ErrOld = Err;
+      } else {
+        Fac = MIN(FacMax, F90MAX(FacMin, FacSafe / pow(Err, ONE / m->ELO)));
+      }
+      Hnew = H * Fac;
+      c->st.Nstp++;
+      if ((Err <= ONE) || (H <= Hmin)) {
+        if (c->Hacc1 == ZERO) c->Hacc1 = H;
+        c->st.Nacc++;
+        if (o->nonneg)
+          for (i = 0; i < N; i++) Y[i] = F90MAX(Ynew[i], ZERO);
+        else
+          for (i = 0; i < N; i++) Y[i] = Ynew[i];
+        T = T + Direction * H;
+        Hnew = F90MAX(Hmin, MIN(Hnew, Hmax));
+        if (RejectLastH) Hnew = MIN(Hnew, H);
+        rstatus[1] = H;
+        rstatus[2] = Hnew;
+        rstatus[0] = T;
+        RejectLastH = 0;
+        RejectMoreH = 0;
+        H = Hnew;
+        Nrun = 0;
+        break;
+      } else {
+        if (RejectMoreH) Hnew = H * FacRej;
+        RejectMoreH = RejectLastH;
+        RejectLastH = 1;
+        H = Hnew;
+        if (c->st.Nacc >= 1) c->st.Nrej++;
+        if (!(fabs(Hnew) <= DBL_MAX)) return IERR_NONFINITE;
+        if (++Nrun > MAX_REJECT) return IERR_REJECTS;
+      }
+    }
+  }
+  return 1;
+}
+
+/* Rosenbrock, int/rosenbrock.f90:128-446: ICNTRL/RCNTRL as merged by
+ * INTEGRATE; istatus[0..7], rstatus[0..2] */
+static int Rosenbrock_f90(ros_cell* c, double* Y, double Tstart, double Tend,
+                          const double* AbsTol, const double* RelTol, const double* RCNTRL,
+                          const int* ICNTRL, const f90_opts* o, int* istatus, double* rstatus) {
+  ros_method m;
+  int Max_no_steps, UplimTol, Autonomous, VectorTol, IERR;
+  double Roundoff, Hmin, Hmax, Hstart, FacMin, FacMax, FacRej, FacSafe;
+  const double span = fabs(Tend - Tstart);
+  memset(&c->st, 0, sizeof c->st);
+  for (int i = 0; i < 3; i++) rstatus[i] = ZERO;
+  Autonomous = !(ICNTRL[0] == 0);
+  if (ICNTRL[1] == 0) { VectorTol = 1; UplimTol = KPP_NVAR; }
+  else { VectorTol = 0; UplimTol = 1; }
+  IERR = 0;
+  if (!method_f90(ICNTRL[2], &m)) IERR = -2;
+  if (!IERR) {
+    if (ICNTRL[3] == 0) Max_no_steps = 200000;
+    else if (ICNTRL[3] > 0) Max_no_steps = ICNTRL[3];
+    else IERR = -1;
+  }
+  Roundoff = 2.220446049250313e-16; /* EPSILON(0.0_dp) = 2^-52 */
+  if (!IERR) {
+    if (RCNTRL[0] == ZERO) Hmin = ZERO;
+    else if (RCNTRL[0] > ZERO) Hmin = RCNTRL[0];
+    else IERR = -3;
+  }
+  if (!IERR) {
+    if (RCNTRL[1] == ZERO) Hmax = span;
+    else if (RCNTRL[1] > ZERO) Hmax = MIN(fabs(RCNTRL[1]), span);
+    else IERR = -3;
+  }
+  if (!IERR) {
+    if (RCNTRL[2] == ZERO) Hstart = F90MAX(Hmin, DeltaMinF90);
+    else if (RCNTRL[2] > ZERO) Hstart = MIN(fabs(RCNTRL[2]), span);
+    else IERR = -3;
+  }
+  if (!IERR) {
+    const double def[4] = {0.2, 6.0, 0.1, 0.9};
+    double* dst[4] = {&FacMin, &FacMax, &FacRej, &FacSafe};
+    for (int k = 0; k < 4 && !IERR; k++) {
+      if (RCNTRL[3 + k] == ZERO) *dst[k] = def[k];
+      else if (RCNTRL[3 + k] > ZERO) *dst[k] = RCNTRL[3 + k];
+      else IERR = -4;
+    }
+  }
+  if (!IERR)
+    for (int i = 0; i < UplimTol; i++)
+      if ((AbsTol[i] <= ZERO) || (RelTol[i] <= 10.0 * Roundoff) || (RelTol[i] >= ONE)) {
+        IERR = -5;
+        break;
+      }
+  if (!IERR)
+    IERR = ros_Integrator_f90(c, Y, Tstart, Tend, AbsTol, RelTol, &m, Autonomous, VectorTol,
+                              Max_no_steps, Roundoff, Hmin, Hmax, Hstart, FacMin, FacMax, FacRej,
+                              FacSafe, o, rstatus);
+  istatus[0] = c->st.Nfun; istatus[1] = c->st.Njac; istatus[2] = c->st.Nstp;
+  istatus[3] = c->st.Nacc; istatus[4] = c->st.Nrej; istatus[5] = c->st.Ndec;
+  istatus[6] = c->st.Nsol; istatus[7] = c->st.Nsng;
+  return IERR;
+}
+
+/* The merge of INTEGRATE (int/rosenbrock.f90:57-80): ICNTRL = 0 with
+ * ICNTRL(15) = 5, then ICNTRL_U where /= 0; RCNTRL = 0, then RCNTRL_U where
+ * > 0.  (Either user array may be NULL: not PRESENT.) */
+void ros_ref_f90_merge(const int* ICNTRL_U, const double* RCNTRL_U, int* ICNTRL,
+                       double* RCNTRL) {
+  for (int i = 0; i < 20; i++) { ICNTRL[i] = 0; RCNTRL[i] = ZERO; }
+  ICNTRL[14] = 5;
+  for (int i = 0; i < 20; i++) {
+    if (ICNTRL_U && ICNTRL_U[i] != 0) ICNTRL[i] = ICNTRL_U[i];
+    if (RCNTRL_U && RCNTRL_U[i] > ZERO) RCNTRL[i] = RCNTRL_U[i];
+  }
+}
+
+/* Batched F90 INTEGRATE over SoA concentrations, cell c over [tin[c], tout[c]].
+ * ICNTRL_U/RCNTRL_U as the F90 caller passes them (merged as INTEGRATE does);
+ * ICNTRL(15) selects the rate updates (util/util.f90:133-159): RCONST bit 1
+ * -> rates recomputed at every Fun/Jac (with Update_SUN from TIME if bit 4,
+ * else the cell's SUN), otherwise the cell's rates stay as computed at entry
+ * (or as given in rconst).  passive: [NVAR] or NULL.  istatus [8][ncells],
+ * rstatus [3][ncells] (Texit, Hexit, Hnew), ierr [ncells]. */
+void ros_ref_f90_batch(int ncells, double* C, const double* temp, const double* cfactor,
+                       const double* sun, const double* tin, const double* tout,
+                       const double* atol, const double* rtol, const int* ICNTRL_U,
+                       const double* RCNTRL_U, const int* passive, const double* rconst,
+                       int* ierr, int* istatus, double* rstatus, int h211b, int nthreads) {
+  int ICNTRL[20];
+  double RCNTRL[20];
+  ros_ref_f90_merge(ICNTRL_U, RCNTRL_U, ICNTRL, RCNTRL);
+  const int opt = ICNTRL[14];
+  const int do_rconst = (opt != -1) && (opt & 1), do_sun = (opt != -1) && (opt & 4);
+  f90_opts o;
+  o.passive = passive;
+  o.nonneg = (ICNTRL[15] == 1);
+  o.np_count = KPP_NVAR;
+  /* h211b: the integrator of int/rosenbrock_h211b_qssa.f90 (its QSSA
+   * options, ICNTRL(18), are not restated); H211b control when RCNTRL(15)
+   * and RCNTRL(16) are both set (:490-514), else the F90 controller */
+  o.bb = RCNTRL[14];
+  o.kk = RCNTRL[15];
+  o.h211b = h211b && o.bb > ZERO && o.kk > ZERO;
+  if (passive) {
+    o.np_count = 0;
+    for (int i = 0; i < KPP_NVAR; i++) o.np_count += passive[i] ? 0 : 1;
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+  for (int cidx = 0; cidx < ncells; cidx++) {
+    ros_cell c;
+    double Y[KPP_NVAR];
+    int ist[8];
+    double rst[3];
+    memset(&c, 0, sizeof c);
+    c.TEMP = temp[cidx];
+    c.CFACTOR = cfactor[cidx];
+    c.SUN = sun ? sun[cidx] : 0.0;
+    c.sun_mode = (do_rconst && do_sun) ? 1 : 0;
+    c.fun_split = KPP_FUN_SPLIT;
+    for (int i = 0; i < KPP_NVAR; i++) Y[i] = C[(size_t)i * ncells + cidx];
+    for (int i = 0; i < KPP_NFIX; i++) c.FIX[i] = C[(size_t)(KPP_NVAR + i) * ncells + cidx];
+    if (rconst) {
+      c.rates_given = 1;
+      for (int i = 0; i < KPP_NREACT; i++) c.RCONST[i] = rconst[(size_t)i * ncells + cidx];
+    } else if (!do_rconst) {
+      /* rates the caller's Update_RCONST left before INTEGRATE: computed
+       * once from the cell's state at entry, then held */
+      kpp_Update_RCONST(c.TEMP, c.SUN, c.CFACTOR, tin[cidx], Y, c.FIX, c.RCONST);
+      c.rates_given = 1;
+    }
+    int e = Rosenbrock_f90(&c, Y, tin[cidx], tout[cidx], atol, rtol, RCNTRL, ICNTRL, &o, ist,
+                           rst);
+    for (int i = 0; i < KPP_NVAR; i++) C[(size_t)i * ncells + cidx] = Y[i];
+    ierr[cidx] = e;
+    if (istatus)
+      for (int i = 0; i < 8; i++) istatus[(size_t)i * ncells + cidx] = ist[i];
+    if (rstatus)
+      for (int i = 0; i < 3; i++) rstatus[(size_t)i * ncells + cidx] = rst[i];
+  }
+}
+
+/* the C SDIRK integrator (int/sdirk.c) on the same model functions */
+#include "sdirk_ref.h"
+#include "rk_ref.h"
+/* the C discrete adjoint Rosenbrock integrator (int/rosenbrock_adj.c) */
+#if defined(KPP_ADJOINT) && KPP_ADJOINT
+#include "rosadj_ref.h"
+#endif
+
 /* ======================= exported entry points ======================= */
 int ros_ref_sizes(int* out) {
   out[0] = KPP_NVAR;
@@ -608,6 +1010,16 @@ void ros_ref_jac_sp(const double* V, const double* F, const double* RCT, double*
   kpp_Jac_SP(V, F, RCT, JVS);
 }
 int ros_ref_decomp(double* JVS) { return KppDecomp(JVS); }
+/* Fun_SPLIT for tests: 1 and Vdot = P - D*V for a #FUNCTION SPLIT mechanism, else 0 */
+int ros_ref_fun_split(const double* V, const double* F, const double* RCT, double* Vdot) {
+#if KPP_FUN_SPLIT
+  kpp_Fun_SPLIT(V, F, RCT, Vdot);
+  return 1;
+#else
+  (void)V; (void)F; (void)RCT; (void)Vdot;
+  return 0;
+#endif
+}
 void ros_ref_solve(const double* JVS, double* X) { kpp_KppSolve(JVS, X); }
 /* Ghimj build + decomposition of ros_PrepareMatrix for one fixed H */
 int ros_ref_prepare(const double* Jac0, double H, double gam, double* Ghimj) {

@@ -287,10 +287,10 @@ def test_kppgen_compiles_plain_mechanism(tmp_path):
 
 
 @pytest.mark.parametrize("extra,why", [
-    ("#FAMILIES\nFX : X + Y;\n", "FAMILIES"),
+    ("#FAMILIES\nFX : X + Q;\n", "FX"),  # a member that is no species
     ("#INLINE C_RCONST\n  RCONST[0] = 1.0;\n#ENDINLINE\n", "C_RCONST"),
     ("#INLINE C_RATES\ndouble k(double t) { return t; }\n#ENDINLINE\n", "C_RATES"),
-    ("#FUNCTION SPLIT\n", "FUNCTION"),
+    ("#FUNCTION SPARSE\n", "FUNCTION"),
     ("#JACOBIAN FULL\n", "JACOBIAN"),
     ("#DOUBLE OFF\n", "DOUBLE"),
     ("#FLUX ON\n", "FLUX"),
