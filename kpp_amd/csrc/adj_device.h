@@ -210,10 +210,9 @@ struct AdjCell {
     A J0 = at(L::JAC0), G = at(L::GHIMJ);
     int nconsecutive = 0;
     for (;;) {
-      for (int i = 0; i < M::LU_NONZERO; i++) G[i] = J0[i];
-      wscal(M::LU_NONZERO, -1.0, G);
+      // Ghimj = -Jac0, then ghinv added on the diagonal
       const double ghinv = 1.0 / (s.Direction * H * p.coef.Gamma[0]);
-      call_diag_fill<M>(G.b, ghinv);
+      call_jac_to_ghimj<M>(J0.b, ghinv, G.b);
       const int ising = call_decomp_checked_g<M>(G.b);
       s.Ndec++;
       if (ising == 0) return false;
@@ -354,10 +353,9 @@ struct AdjCell {
       A Yst0 = ck(k, 2);
       A G = p.adj.save_lu ? ck(k, 2 + 2 * m.S * N) : at(L::GHIMJ);
       if (!p.adj.save_lu) {
-        jac(p, s, T, Yst0, G);
+        jac(p, s, T, Yst0, Jac);
         const double Tau = 1.0 / (s.Direction * H * m.Gamma[0]);
-        wscal(M::LU_NONZERO, -1.0, G);
-        call_diag_fill<M>(G.b, Tau);
+        call_jac_to_ghimj<M>(Jac.b, Tau, G.b);
         (void)call_decomp_checked_g<M>(G.b);
         s.Ndec++;
       }
