@@ -35,6 +35,7 @@
 #include <functional>
 #include <map>
 #include <set>
+#include <array>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -2634,6 +2635,7 @@ void emit_hip(Model& md, const std::string& path) {
 // descriptor per (step, lane); an all-ones descriptor is an idle lane.
 // ---------------------------------------------------------------------------
 bool g_coop = false;
+bool g_coop_place = true;  // --coop-identity-layout: LU entry n at LDS slot n
 
 namespace coop {
 
@@ -2800,7 +2802,8 @@ uint64_t op_desc(int a, int b, int t, bool first, bool last, bool div) {
 
 // ---- the device encoding of a scheduled program ----
 // A cell's LDS (byte addresses; the kernel's dynamic LDS starts at 0):
-//   G region [0, GZ), GZ = 8*LU_NONZERO: the LU entries;
+//   G region [0, GZ), GZ = 8*LU_NONZERO: the LU entries, entry n at slot
+//   GSLOT[n] (place_lds);
 //   T region from GZ: T-relative 0 is a junk slot (JUNK), 8 holds +0.0
 //   (ZERO), from 16 the products of Fun/Jac_SP (Fun's results after the A
 //   products), the solve's X then Z vectors, the error-norm terms.
@@ -2818,7 +2821,8 @@ constexpr uint32_t T_JUNK = 0, T_ZERO = 8, T_DATA = 16;
 // step: the chain's, 1.0 for the single step of an empty chain (0.0 * 1.0),
 // -0.0 for idle steps.
 void encode_sums(const std::vector<uint64_t>& tab, bool jvs, uint32_t gz, uint32_t fo,
-                 const std::vector<double>& pool, std::vector<uint64_t>& out,
+                 const std::vector<double>& pool, const std::vector<int>& gslot,
+                 const std::vector<int>& pslot, std::vector<uint64_t>& out,
                  std::vector<double>& coef) {
   const size_t steps = tab.size() / 64;
   out.assign(tab.size(), 0);
@@ -2833,9 +2837,9 @@ void encode_sums(const std::vector<uint64_t>& tab, bool jvs, uint32_t gz, uint32
       if (d & VALID) {
         const bool zero = (d >> 50) & 1;
         const uint32_t src = (uint32_t)((d >> 16) & 0xFFFF), tgt = (uint32_t)((d >> 32) & 0xFFFF);
-        p = zero ? T_ZERO : T_DATA + 8 * src;
+        p = zero ? T_ZERO : T_DATA + 8 * (uint32_t)pslot[src];
         c = zero ? 1.0 : pool[d & 0xFFFF];
-        s = jvs ? 8 * tgt : fo + 8 * tgt;
+        s = jvs ? 8 * (uint32_t)gslot[tgt] : fo + 8 * tgt;
         first = (d >> 48) & 1;
         prev = s;
       }
@@ -2852,12 +2856,12 @@ void encode_sums(const std::vector<uint64_t>& tab, bool jvs, uint32_t gz, uint32
 // JUNK when the step is not a chain's first (JUNK is the lowest address of
 // the T region: solve inits are above it, LU inits below it).
 void encode_ops(const std::vector<uint64_t>& tab, bool solve, uint32_t gz, int nvar,
-                std::vector<uint64_t>& out) {
+                const std::vector<int>& gslot, std::vector<uint64_t>& out) {
   const size_t steps = tab.size() / 64;
   out.assign(tab.size(), 0);
   const uint32_t junk = solve ? T_JUNK : gz + T_JUNK;
   const uint32_t zero_a = gz + T_ZERO, zero_b = solve ? T_ZERO : gz + T_ZERO;
-  auto x_off = [&](uint32_t idx) { return solve ? T_DATA + 8 * idx : 8 * idx; };
+  auto x_off = [&](uint32_t idx) { return solve ? T_DATA + 8 * idx : 8 * (uint32_t)gslot[idx]; };
   for (int l = 0; l < 64; l++) {
     uint32_t prev = junk;
     for (size_t t = 0; t < steps; t++) {
@@ -2866,7 +2870,7 @@ void encode_ops(const std::vector<uint64_t>& tab, bool solve, uint32_t gz, int n
       if (d & VALID) {
         const uint32_t ai = (uint32_t)(d & 0xFFFF), bi = (uint32_t)((d >> 16) & 0xFFFF),
                        ti = (uint32_t)((d >> 32) & 0xFFFF);
-        a = 8 * ai;
+        a = 8 * (uint32_t)gslot[ai];
         b = x_off(bi);
         s = x_off(ti);
         if ((d >> 48) & 1) {
@@ -2887,6 +2891,73 @@ void encode_ops(const std::vector<uint64_t>& tab, bool solve, uint32_t gz, int n
     if (any)
       for (int l = 0; l < 64; l++) out[t * 64 + l] |= 2ull << 32;
   }
+}
+
+// Placement of LDS arrays (the LU entries, the Fun and Jac_SP products).  A
+// step's 64 lanes read up to 64 distinct elements per operand; elements whose
+// 8-byte slots fall in the same bank pair are served one after the other.
+// Each element gets a colour (its slot modulo BANKS8, the doubles per bank
+// row) so that the elements one instruction touches spread over the colours:
+// greedily, in the order of first use, the colour with the fewest elements
+// of the element's instructions already on it, among the colours with free
+// slots.  Returns slot[element], a permutation of 0..nnz-1.
+constexpr int BANKS8 = 32;
+std::vector<int> place_lds(int nnz, const std::vector<const std::vector<uint64_t>*>& progs,
+                          const std::vector<int>& fields_of) {
+  // instructions: (program, step, field) -> the distinct entries it touches
+  std::vector<std::vector<int>> instr;
+  std::vector<std::vector<int>> of_entry(nnz);
+  std::vector<int> order;
+  std::vector<char> seen(nnz, 0);
+  for (size_t pi = 0; pi < progs.size(); pi++) {
+    const auto& tab = *progs[pi];
+    const size_t steps = tab.size() / 64;
+    for (size_t t = 0; t < steps; t++)
+      for (int f = 0; f < 3; f++) {
+        if (!((fields_of[pi] >> f) & 1)) continue;
+        std::set<int> es;
+        for (int l = 0; l < 64; l++) {
+          const uint64_t d = tab[t * 64 + l];
+          if (d & VALID) es.insert((int)((d >> (16 * f)) & 0xFFFF));
+        }
+        if (es.size() < 2) {
+          for (int e : es)
+            if (!seen[e]) { seen[e] = 1; order.push_back(e); }
+          continue;
+        }
+        const int id = (int)instr.size();
+        instr.emplace_back(es.begin(), es.end());
+        for (int e : es) {
+          of_entry[e].push_back(id);
+          if (!seen[e]) { seen[e] = 1; order.push_back(e); }
+        }
+      }
+  }
+  for (int e = 0; e < nnz; e++)
+    if (!seen[e]) order.push_back(e);
+  std::vector<int> cap(BANKS8), used(BANKS8, 0), colour(nnz, -1);
+  for (int c = 0; c < BANKS8; c++) cap[c] = nnz / BANKS8 + (c < nnz % BANKS8 ? 1 : 0);
+  std::vector<std::array<int, BANKS8>> cnt(instr.size());
+  for (auto& a : cnt) a.fill(0);
+  for (int e : order) {
+    int best = -1;
+    long bc = 0;
+    for (int c = 0; c < BANKS8; c++) {
+      if (used[c] >= cap[c]) continue;
+      long cost = 0;
+      for (int id : of_entry[e]) cost += cnt[id][c];
+      if (best < 0 || cost < bc || (cost == bc && cap[c] - used[c] > cap[best] - used[best])) {
+        best = c;
+        bc = cost;
+      }
+    }
+    colour[e] = best;
+    for (int id : of_entry[e]) cnt[id][best]++;
+    used[best]++;
+  }
+  std::vector<int> slot(nnz), next(BANKS8, 0);
+  for (int e = 0; e < nnz; e++) slot[e] = colour[e] + BANKS8 * next[colour[e]]++;
+  return slot;
 }
 
 void emit_table(std::ostream& o, const char* type, const std::string& name, size_t n,
@@ -3072,12 +3143,67 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
       return std::to_string(v) + "u";
     };
   };
+  // LU entries' LDS slots: the LU program reads fields a, b and its
+  // targets (bits 0-15, 16-31, 32-47), the solve field a, Jac_SP writes targets
+  const std::vector<int> gslot =
+      g_coop_place ? place_lds(nnz, {&tlu, &tsol, &tjac}, {7, 1, 4}) : [&] {
+        std::vector<int> v(nnz);
+        for (int n = 0; n < nnz; n++) v[n] = n;
+        return v;
+      }();
+  {
+    // the mean over instructions touching two or more entries of the
+    // largest number of distinct entries on one bank pair
+    auto load = [&](const std::vector<int>& sl) {
+      const std::vector<const std::vector<uint64_t>*> progs = {&tlu, &tsol, &tjac};
+      const int fields[] = {7, 1, 4};
+      double tot = 0.0;
+      long n = 0;
+      for (size_t pi = 0; pi < progs.size(); pi++) {
+        const auto& tab = *progs[pi];
+        for (size_t t = 0; t < tab.size() / 64; t++)
+          for (int f = 0; f < 3; f++) {
+            if (!((fields[pi] >> f) & 1)) continue;
+            std::set<int> es;
+            for (int l = 0; l < 64; l++)
+              if (tab[t * 64 + l] & VALID) es.insert((int)((tab[t * 64 + l] >> (16 * f)) & 0xFFFF));
+            if (es.size() < 2) continue;
+            int c[BANKS8] = {0}, mx = 0;
+            for (int e : es) mx = std::max(mx, ++c[sl[e] % BANKS8]);
+            tot += mx;
+            n++;
+          }
+      }
+      return n ? tot / n : 0.0;
+    };
+    std::vector<int> ident(nnz);
+    for (int k = 0; k < nnz; k++) ident[k] = k;
+    fprintf(stderr, "kppgen: %s coop LU bank-pair load per instruction: identity %.2f, placed %.2f\n",
+            md.m->root.c_str(), load(ident), load(gslot));
+  }
+  // the products' slots (the sum programs read field b, bits 16-31): the
+  // product tables are written in slot order, so the device's product loop
+  // (lane-strided over slots) needs no map
+  auto ident_of = [](int n) {
+    std::vector<int> v(n);
+    for (int k = 0; k < n; k++) v[k] = k;
+    return v;
+  };
+  const std::vector<int> aslot = g_coop_place ? place_lds(na, {&tfun}, {2}) : ident_of(na);
+  const std::vector<int> bslot = g_coop_place ? place_lds(nb, {&tjac}, {2}) : ident_of(nb);
+  {
+    std::vector<std::vector<uint32_t>> a2(na), b2(nb);
+    for (int q = 0; q < na; q++) a2[aslot[q]] = aprod[q];
+    for (int q = 0; q < nb; q++) b2[bslot[q]] = bprod[q];
+    aprod.swap(a2);
+    bprod.swap(b2);
+  }
   std::vector<uint64_t> efun, ejac, elu, esol;
   std::vector<double> cfun, cjac;
-  encode_sums(tfun, false, gz, fo, pool, efun, cfun);
-  encode_sums(tjac, true, gz, fo, pool, ejac, cjac);
-  encode_ops(tlu, false, gz, nv, elu);
-  encode_ops(tsol, true, gz, nv, esol);
+  encode_sums(tfun, false, gz, fo, pool, gslot, aslot, efun, cfun);
+  encode_sums(tjac, true, gz, fo, pool, gslot, bslot, ejac, cjac);
+  encode_ops(tlu, false, gz, nv, gslot, elu);
+  encode_ops(tsol, true, gz, nv, gslot, esol);
   auto dbl = [](double v) {
     if (v == 0.0) return std::string(std::signbit(v) ? "-0.0" : "0.0");
     char b[40];
@@ -3101,7 +3227,9 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   emit_table(o, "double", "JAC_COEF", cjac.size(), [&](size_t i) { return dbl(cjac[i]); });
   emit_table(o, "unsigned long long", "LU", elu.size(), [&](size_t i) { return hex(elu[i]); });
   emit_table(o, "unsigned long long", "SOL", esol.size(), [&](size_t i) { return hex(esol[i]); });
-  emit_table(o, "unsigned", "DIAG", nv, [&](size_t i) { return std::to_string(S.lu_diag[i]) + "u"; });
+  emit_table(o, "unsigned", "DIAG", nv,
+             [&](size_t i) { return std::to_string(gslot[S.lu_diag[i]]) + "u"; });
+  emit_table(o, "unsigned", "GSLOT", nnz, [&](size_t i) { return std::to_string(gslot[i]) + "u"; });
   o << "}  // namespace " << ns << "_coop_data\n";
   o << "struct " << ns << "_coop {\n";
   o << "static constexpr bool ENABLED = true;\n";
@@ -3115,11 +3243,11 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   o << "static constexpr int FUN_STEPS = " << tf << ", JAC_STEPS = " << tj
     << ", LU_STEPS = " << tl << ", SOL_STEPS = " << ts << ";\n";
   const char* names[] = {"APROD", "BPROD", "CONSTS", "FUN", "JAC", "LU", "SOL", "DIAG",
-                         "FUN_COEF", "JAC_COEF"};
+                         "FUN_COEF", "JAC_COEF", "GSLOT"};
   const char* types[] = {"unsigned", "unsigned", "double", "unsigned long long",
                          "unsigned long long", "unsigned long long", "unsigned long long",
-                         "unsigned", "double", "double"};
-  for (int k = 0; k < 10; k++)
+                         "unsigned", "double", "double", "unsigned"};
+  for (int k = 0; k < 11; k++)
     o << "static __device__ __forceinline__ const " << types[k] << "* " << names[k]
       << "() { return " << ns << "_coop_data::" << names[k] << "; }\n";
   o << "};\n";
@@ -3240,6 +3368,7 @@ int main(int argc, char** argv) {
     else if (a == "--no-pipelined") g_emit_pipelined = false;
     else if (a == "--pipelined") g_emit_pipelined = true;
     else if (a == "--coop") g_coop = true;
+    else if (a == "--coop-identity-layout") g_coop_place = false;
     else if (a == "--part-ops") g_part_ops = atol(next().c_str());
     else if (a == "--lds-split") g_lds_split = atoi(next().c_str());
     else if (a == "--f90-module") f90_modules.push_back(next());

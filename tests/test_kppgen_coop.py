@@ -136,7 +136,7 @@ extern "C" void coop_jac(double* V, double* F, double* R, double ghinv, double* 
   products(C::BPROD(), C::NB, C::NB_PAD, V, F, R);
   sums(C::JAC(), C::JAC_COEF(), C::JAC_STEPS, true);
   for (int i = 0; i < M::NVAR; i++) at(8 * C::DIAG()[i]) = at(8 * C::DIAG()[i]) + ghinv;
-  for (int n = 0; n < M::LU_NONZERO; n++) G[n] = at(8 * n);
+  for (int n = 0; n < M::LU_NONZERO; n++) G[n] = at(8 * C::GSLOT()[n]);
 }
 // an elimination program in lockstep: at step t every lane reads the
 // operands of step t+1, then computes step t and stores
@@ -173,9 +173,9 @@ static void program(const unsigned long long* prog, int steps, bool solve) {
 }
 extern "C" void coop_lu(double* G, double* X) {
   fresh();
-  for (int n = 0; n < M::LU_NONZERO; n++) at(8 * n) = G[n];
+  for (int n = 0; n < M::LU_NONZERO; n++) at(8 * C::GSLOT()[n]) = G[n];
   program(C::LU(), C::LU_STEPS, false);
-  for (int n = 0; n < M::LU_NONZERO; n++) G[n] = at(8 * n);
+  for (int n = 0; n < M::LU_NONZERO; n++) G[n] = at(8 * C::GSLOT()[n]);
   for (int i = 0; i < M::NVAR; i++) at(C::GZ + 16 + 8 * i) = X[i];
   program(C::SOL(), C::SOL_STEPS, true);
   for (int i = 0; i < M::NVAR; i++) X[i] = at(C::GZ + 16 + 8 * (M::NVAR + i));
