@@ -2717,12 +2717,34 @@ bool parse_product(const std::string& e, std::vector<uint32_t>& fac, std::vector
 
 struct Chain {
   std::vector<uint64_t> steps;          // descriptors
+  std::vector<uint64_t> steps2;         // paired programs: each step's second term (or IDLE)
   std::vector<std::vector<int>> deps;   // per step: chains whose results it reads
   int init_dep = -1;                    // chain producing the initial value (first step)
 };
 
+// An elimination chain as a paired program: steps 2k and 2k+1 become one
+// step with two terms (the second IDLE for an odd chain's last), reading the
+// operands of both.  Same operations in the same order; half the stores.
+Chain pair_chain(const Chain& c) {
+  Chain p;
+  p.init_dep = c.init_dep;
+  for (size_t k = 0; k < c.steps.size(); k += 2) {
+    p.steps.push_back(c.steps[k]);
+    std::vector<int> d = c.deps[k];
+    if (k + 1 < c.steps.size()) {
+      p.steps2.push_back(c.steps[k + 1]);
+      d.insert(d.end(), c.deps[k + 1].begin(), c.deps[k + 1].end());
+    } else {
+      p.steps2.push_back(IDLE);
+    }
+    p.deps.push_back(d);
+  }
+  return p;
+}
+
 // List scheduling on 64 lanes; returns the table [T][64].
-std::vector<uint64_t> schedule(const std::vector<Chain>& ch, int dist, int& T) {
+std::vector<uint64_t> schedule(const std::vector<Chain>& ch, int dist, int& T,
+                               std::vector<uint64_t>* tab2 = nullptr) {
   const int n = (int)ch.size();
   std::vector<std::vector<int>> succ(n);
   for (int c = 0; c < n; c++) {
@@ -2779,6 +2801,12 @@ std::vector<uint64_t> schedule(const std::vector<Chain>& ch, int dist, int& T) {
   for (int c = 0; c < n; c++)
     for (size_t k = 0; k < ch[c].steps.size(); k++)
       tab[(size_t)(at[c].start + k) * 64 + at[c].lane] = ch[c].steps[k];
+  if (tab2) {
+    tab2->assign(tab.size(), IDLE);
+    for (int c = 0; c < n; c++)
+      for (size_t k = 0; k < ch[c].steps2.size(); k++)
+        (*tab2)[(size_t)(at[c].start + k) * 64 + at[c].lane] = ch[c].steps2[k];
+  }
   return tab;
 }
 
@@ -2849,24 +2877,29 @@ void encode_sums(const std::vector<uint64_t>& tab, bool jvs, uint32_t gz, uint32
   }
 }
 
-// LU (every field in G) and solve (a in G; b, target and init in T, X at
-// T_DATA, Z after it) programs: lo = a | b << 16; hi = target | init << 16,
-// target bit 0 = this lane divides (acc / [a]), bit 1 = some lane of the step
-// divides (the same in all lanes); init = the first step's initial value, or
-// JUNK when the step is not a chain's first (JUNK is the lowest address of
-// the T region: solve inits are above it, LU inits below it).
-void encode_ops(const std::vector<uint64_t>& tab, bool solve, uint32_t gz, int nvar,
-                const std::vector<int>& gslot, std::vector<uint64_t>& out) {
+// Paired LU (every field in G) and solve (a in G; b, target and init in T,
+// X at T_DATA, Z after it) programs, two words per (step, lane): lo = a1 |
+// b1 << 16 | a2 << 32 | b2 << 48 (the step's two terms; an absent second
+// term reads +0.0 twice), hi = target | init << 16, target bit 2 = the first
+// term divides (acc / [a1]), bit 0 = the second does (acc / [a2]), bit 1 =
+// some lane of the step divides (the same in all lanes); init = the first
+// term's initial value when the step starts a chain, else JUNK (JUNK is the
+// lowest address of the T region: solve inits are above it, LU inits below
+// it).
+void encode_ops(const std::vector<uint64_t>& tab, const std::vector<uint64_t>& tab2, bool solve,
+                uint32_t gz, int nvar, const std::vector<int>& gslot, std::vector<uint64_t>& lo,
+                std::vector<uint64_t>& hi) {
   const size_t steps = tab.size() / 64;
-  out.assign(tab.size(), 0);
+  lo.assign(tab.size(), 0);
+  hi.assign(tab.size(), 0);
   const uint32_t junk = solve ? T_JUNK : gz + T_JUNK;
   const uint32_t zero_a = gz + T_ZERO, zero_b = solve ? T_ZERO : gz + T_ZERO;
   auto x_off = [&](uint32_t idx) { return solve ? T_DATA + 8 * idx : 8 * (uint32_t)gslot[idx]; };
   for (int l = 0; l < 64; l++) {
     uint32_t prev = junk;
     for (size_t t = 0; t < steps; t++) {
-      const uint64_t d = tab[t * 64 + l];
-      uint32_t a = zero_a, b = zero_b, s = prev, init = junk, div = 0;
+      const uint64_t d = tab[t * 64 + l], d2 = tab2[t * 64 + l];
+      uint32_t a = zero_a, b = zero_b, a2 = zero_a, b2 = zero_b, s = prev, init = junk, flags = 0;
       if (d & VALID) {
         const uint32_t ai = (uint32_t)(d & 0xFFFF), bi = (uint32_t)((d >> 16) & 0xFFFF),
                        ti = (uint32_t)((d >> 32) & 0xFFFF);
@@ -2878,18 +2911,25 @@ void encode_ops(const std::vector<uint64_t>& tab, bool solve, uint32_t gz, int n
           const uint32_t ii = (solve && ti >= (uint32_t)nvar) ? ti - (uint32_t)nvar : ti;
           init = x_off(ii);
         }
-        div = (d >> 50) & 1;
+        if ((d >> 50) & 1) flags |= 4;
+        if (d2 & VALID) {
+          if (((d2 >> 32) & 0xFFFF) != ti || ((d2 >> 48) & 1)) die("coop: a pair spans two chains");
+          a2 = 8 * (uint32_t)gslot[d2 & 0xFFFF];
+          b2 = x_off((uint32_t)((d2 >> 16) & 0xFFFF));
+          if ((d2 >> 50) & 1) flags |= 1;
+        }
         prev = s;
       }
-      out[t * 64 + l] = (uint64_t)a | ((uint64_t)b << 16) | ((uint64_t)(s | div) << 32) |
-                        ((uint64_t)init << 48);
+      lo[t * 64 + l] = (uint64_t)a | ((uint64_t)b << 16) | ((uint64_t)a2 << 32) |
+                       ((uint64_t)b2 << 48);
+      hi[t * 64 + l] = (uint64_t)(s | flags) | ((uint64_t)init << 16);
     }
   }
   for (size_t t = 0; t < steps; t++) {
     bool any = false;
-    for (int l = 0; l < 64; l++) any |= (out[t * 64 + l] >> 32) & 1;
+    for (int l = 0; l < 64; l++) any |= (hi[t * 64 + l] & 5) != 0;
     if (any)
-      for (int l = 0; l < 64; l++) out[t * 64 + l] |= 2ull << 32;
+      for (int l = 0; l < 64; l++) hi[t * 64 + l] |= 2;
   }
 }
 
@@ -3126,8 +3166,11 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
     return;
   }
   int tf = 0, tj = 0, tl = 0, ts = 0;
-  auto tfun = schedule(fun, 2, tf), tjac = schedule(jac, 2, tj), tlu = schedule(lu, 2, tl),
-       tsol = schedule(sol, 2, ts);
+  for (auto& c : lu) c = pair_chain(c);
+  for (auto& c : sol) c = pair_chain(c);
+  std::vector<uint64_t> tlu2, tsol2;
+  auto tfun = schedule(fun, 2, tf), tjac = schedule(jac, 2, tj), tlu = schedule(lu, 2, tl, &tlu2),
+       tsol = schedule(sol, 2, ts, &tsol2);
   auto hex = [](uint64_t v) {
     char b[32];
     snprintf(b, sizeof b, "0x%llxull", (unsigned long long)v);
@@ -3146,7 +3189,7 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   // LU entries' LDS slots: the LU program reads fields a, b and its
   // targets (bits 0-15, 16-31, 32-47), the solve field a, Jac_SP writes targets
   const std::vector<int> gslot =
-      g_coop_place ? place_lds(nnz, {&tlu, &tsol, &tjac}, {7, 1, 4}) : [&] {
+      g_coop_place ? place_lds(nnz, {&tlu, &tlu2, &tsol, &tsol2, &tjac}, {7, 3, 1, 1, 4}) : [&] {
         std::vector<int> v(nnz);
         for (int n = 0; n < nnz; n++) v[n] = n;
         return v;
@@ -3155,8 +3198,8 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
     // the mean over instructions touching two or more entries of the
     // largest number of distinct entries on one bank pair
     auto load = [&](const std::vector<int>& sl) {
-      const std::vector<const std::vector<uint64_t>*> progs = {&tlu, &tsol, &tjac};
-      const int fields[] = {7, 1, 4};
+      const std::vector<const std::vector<uint64_t>*> progs = {&tlu, &tlu2, &tsol, &tsol2, &tjac};
+      const int fields[] = {7, 3, 1, 1, 4};
       double tot = 0.0;
       long n = 0;
       for (size_t pi = 0; pi < progs.size(); pi++) {
@@ -3202,8 +3245,9 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   std::vector<double> cfun, cjac;
   encode_sums(tfun, false, gz, fo, pool, gslot, aslot, efun, cfun);
   encode_sums(tjac, true, gz, fo, pool, gslot, bslot, ejac, cjac);
-  encode_ops(tlu, false, gz, nv, gslot, elu);
-  encode_ops(tsol, true, gz, nv, gslot, esol);
+  std::vector<uint64_t> eluh, esolh;
+  encode_ops(tlu, tlu2, false, gz, nv, gslot, elu, eluh);
+  encode_ops(tsol, tsol2, true, gz, nv, gslot, esol, esolh);
   auto dbl = [](double v) {
     if (v == 0.0) return std::string(std::signbit(v) ? "-0.0" : "0.0");
     char b[40];
@@ -3227,6 +3271,8 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   emit_table(o, "double", "JAC_COEF", cjac.size(), [&](size_t i) { return dbl(cjac[i]); });
   emit_table(o, "unsigned long long", "LU", elu.size(), [&](size_t i) { return hex(elu[i]); });
   emit_table(o, "unsigned long long", "SOL", esol.size(), [&](size_t i) { return hex(esol[i]); });
+  emit_table(o, "unsigned long long", "LUH", eluh.size(), [&](size_t i) { return hex(eluh[i]); });
+  emit_table(o, "unsigned long long", "SOLH", esolh.size(), [&](size_t i) { return hex(esolh[i]); });
   emit_table(o, "unsigned", "DIAG", nv,
              [&](size_t i) { return std::to_string(gslot[S.lu_diag[i]]) + "u"; });
   emit_table(o, "unsigned", "GSLOT", nnz, [&](size_t i) { return std::to_string(gslot[i]) + "u"; });
@@ -3243,11 +3289,12 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   o << "static constexpr int FUN_STEPS = " << tf << ", JAC_STEPS = " << tj
     << ", LU_STEPS = " << tl << ", SOL_STEPS = " << ts << ";\n";
   const char* names[] = {"APROD", "BPROD", "CONSTS", "FUN", "JAC", "LU", "SOL", "DIAG",
-                         "FUN_COEF", "JAC_COEF", "GSLOT"};
+                         "FUN_COEF", "JAC_COEF", "GSLOT", "LUH", "SOLH"};
   const char* types[] = {"unsigned", "unsigned", "double", "unsigned long long",
                          "unsigned long long", "unsigned long long", "unsigned long long",
-                         "unsigned", "double", "double", "unsigned"};
-  for (int k = 0; k < 11; k++)
+                         "unsigned", "double", "double", "unsigned", "unsigned long long",
+                         "unsigned long long"};
+  for (int k = 0; k < 13; k++)
     o << "static __device__ __forceinline__ const " << types[k] << "* " << names[k]
       << "() { return " << ns << "_coop_data::" << names[k] << "; }\n";
   o << "};\n";

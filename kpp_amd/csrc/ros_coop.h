@@ -192,58 +192,73 @@ __device__ __noinline__ void coop_sums(const unsigned long long* progg, const do
 
 // An elimination program over LDS: LU (every field in G) or the triangular
 // solves (a in G; b, target and init in T, forward results in X = T[0:NVAR),
-// backward in Z = T[NVAR:2NVAR)).  Descriptor: lo = a | b << 16; hi =
-// target | init << 16; target bit 0: this lane divides, bit 1: some lane of
-// the step does (the same in every lane); init = JUNK (the first address of
-// T) unless the step is a chain's first: a solve's inits lie above JUNK, the
-// LU's (in G) below it, so one comparison of the whole word tells.  Step: v = first ? X[init] : acc; v = divides ?
-// v / G[a] : v - G[a]*X[b]; X[target] = v.  The LDS operands of step t+1 are
-// read before step t stores.
+// backward in Z = T[NVAR:2NVAR)), two terms of a chain per step.
+// Descriptor pair: lo = a1 | b1 << 16 | a2 << 32 | b2 << 48; hi = target |
+// init << 16; target bit 2: the first term divides, bit 0: the second
+// does, bit 1: some lane of the step divides (the same in every lane); init
+// = JUNK (the first address of T) unless the step is a chain's first: a
+// solve's inits lie above JUNK, the LU's (in G) below it, so one comparison
+// of the whole word tells.  Step: v = first ? X[init] : acc; v = div1 ?
+// v / G[a1] : v - G[a1]*X[b1]; v = div2 ? v / G[a2] : v - G[a2]*X[b2] (an
+// absent second term reads +0.0 twice); X[target] = v.  The LDS operands of
+// step t+1 are read before step t stores.
 template <bool SOLVE, unsigned GZ>
-__device__ __noinline__ void coop_program(const unsigned long long* progg, int steps, int lane) {
-  gcu64* prog = (gcu64*)progg;
+__device__ __noinline__ void coop_program(const unsigned long long* prog_lo,
+                                          const unsigned long long* prog_hi, int steps,
+                                          int lane) {
   constexpr unsigned XB = SOLVE ? GZ : 0u;  // base of b, target, init
-  unsigned long long r[kRing];
-  gcu64* q = prog + lane;
+  unsigned long long r[kRing], h[kRing];
+  gcu64* q = (gcu64*)prog_lo + lane;
+  gcu64* qh = (gcu64*)prog_hi + lane;
 #pragma unroll
-  for (int k = 0; k < kRing; k++) r[k] = q[k * TILE];
+  for (int k = 0; k < kRing; k++) {
+    r[k] = q[k * TILE];
+    h[k] = qh[k * TILE];
+  }
   q += kRing * TILE;
-  auto operands = [](unsigned long long d, double& ga, double& xb, double& xi) {
-    const unsigned lo = (unsigned)d, hi = (unsigned)(d >> 32);
-    ga = lds_ld(lo & 0xFFFFu);
-    xb = lds_ld(XB + (lo >> 16));
-    xi = lds_ld(XB + (hi >> 16));
+  qh += kRing * TILE;
+  struct Ops {
+    double ga, xb, ga2, xb2, xi;
   };
-  double ga, xb, xi, acc = 0.0;
-  operands(r[0], ga, xb, xi);
+  auto operands = [](unsigned long long d, unsigned long long e) {
+    const unsigned lo = (unsigned)d, lo2 = (unsigned)(d >> 32), hi = (unsigned)e;
+    return Ops{lds_ld(lo & 0xFFFFu), lds_ld(XB + (lo >> 16)), lds_ld(lo2 & 0xFFFFu),
+               lds_ld(XB + (lo2 >> 16)), lds_ld(XB + (hi >> 16))};
+  };
+  double acc = 0.0;
+  Ops cur = operands(r[0], h[0]);
   for (int base = 0; base < steps; base += kRing) {
 #pragma unroll
     for (int k = 0; k < kRing; k++) {
-      const unsigned hi = (unsigned)(r[k] >> 32);
+      const unsigned hi = (unsigned)h[k];
       // step base+k+1: slot k+1, or (k = RING-1) slot 0, refilled at k = 0
-      double gan, xbn, xin;
-      operands(r[(k + 1) % kRing], gan, xbn, xin);
+      const Ops nxt = operands(r[(k + 1) % kRing], h[(k + 1) % kRing]);
       const bool first = SOLVE ? (hi >= 0x10000u) : (hi < (GZ << 16));
-      const double v0 = first ? xi : acc;
-      double v = v0 - ga * xb;
+      const double v0 = first ? cur.xi : acc;
+      double v1 = v0 - cur.ga * cur.xb;
+      double v;
       if (__builtin_amdgcn_readfirstlane(hi) & 2u) {
-        // a step with divisions (the barrier keeps the division out of the
+        // a step with divisions (the barrier keeps the divisions out of the
         // other steps)
         __builtin_amdgcn_sched_barrier(0);
-        if (hi & 1u) v = v0 / ga;
+        if (hi & 4u) v1 = v0 / cur.ga;
+        v = v1 - cur.ga2 * cur.xb2;
+        if (hi & 1u) v = v1 / cur.ga2;
+      } else {
+        v = v1 - cur.ga2 * cur.xb2;
       }
       lds_st(XB + (hi & 0xFFF8u), v);
       acc = v;
-      ga = gan;
-      xb = xbn;
-      xi = xin;
+      cur = nxt;
       // the slot is refilled after its last use, so the load can land in
       // the same registers (no copies, and no wait for the ring, at the
       // round's end)
       __builtin_amdgcn_sched_barrier(0);
       r[k] = q[k * TILE];
+      h[k] = qh[k * TILE];
     }
     q += kRing * TILE;
+    qh += kRing * TILE;
   }
 }
 
@@ -320,12 +335,12 @@ struct CoopCell {
   }
 
   __device__ __forceinline__ void decomp() const {
-    coop_program<false, C::GZ>(C::LU(), C::LU_STEPS, lane);
+    coop_program<false, C::GZ>(C::LU(), C::LUH(), C::LU_STEPS, lane);
     sync();
   }
   // solve G x = T[0:NVAR) in place; the result is left in T[NVAR:2NVAR)
   __device__ __forceinline__ void solve() const {
-    coop_program<true, C::GZ>(C::SOL(), C::SOL_STEPS, lane);
+    coop_program<true, C::GZ>(C::SOL(), C::SOLH(), C::SOL_STEPS, lane);
     sync();
   }
 

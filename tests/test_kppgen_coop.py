@@ -2,8 +2,8 @@
 host by a lockstep emulation of the device interpreters (kpp_amd/csrc/
 ros_coop.h) over the cell's LDS image (byte addresses, G at 0, the T region
 at GZ): 64 lanes; the sum programs read each step's product two steps
-ahead, the elimination programs read the operands of step t+1 before step t
-stores; every step stores, idle steps included, and the device's whole
+ahead, the elimination programs (two terms of a chain per step) read the
+operands of step t+1 before step t stores; every step stores, idle steps included, and the device's whole
 rounds of 8 steps run into the idle padding.  The LDS starts as NaN except
 the +0.0 slot, which must stay +0.0.  A schedule that read a value before it
 is final, an idle step that changed a value, or a table entry naming a wrong
@@ -138,46 +138,50 @@ extern "C" void coop_jac(double* V, double* F, double* R, double ghinv, double* 
   for (int i = 0; i < M::NVAR; i++) at(8 * C::DIAG()[i]) = at(8 * C::DIAG()[i]) + ghinv;
   for (int n = 0; n < M::LU_NONZERO; n++) G[n] = at(8 * C::GSLOT()[n]);
 }
-// an elimination program in lockstep: at step t every lane reads the
-// operands of step t+1, then computes step t and stores
-static void program(const unsigned long long* prog, int steps, bool solve) {
+// an elimination program in lockstep (two terms per step): at step t every
+// lane reads the operands of step t+1, then computes step t and stores
+static void program(const unsigned long long* prog, const unsigned long long* progh, int steps,
+                    bool solve) {
   const unsigned xb = solve ? C::GZ : 0u, noinit = solve ? C::JSLOT : C::GZ + C::JSLOT;
   const int n = rounds(steps);
   if (n + 1 > steps + C::PAD) abort();
-  double acc[64] = {0}, ga[64], xbv[64], xi[64], gn[64], bn[64], in[64];
-  auto load = [&](int t, int l, double& a, double& b, double& c) {
+  struct Ops { double ga, xb, ga2, xb2, xi; };
+  double acc[64] = {0};
+  Ops cur[64], nxt[64];
+  auto load = [&](int t, int l) {
     const unsigned long long e = prog[t * 64 + l];
-    const unsigned lo = (unsigned)e, hi = (unsigned)(e >> 32);
-    a = at(lo & 0xFFFFu);
-    b = at(xb + (lo >> 16));
-    c = at(xb + (hi >> 16));
+    const unsigned lo = (unsigned)e, lo2 = (unsigned)(e >> 32), hi = (unsigned)progh[t * 64 + l];
+    return Ops{at(lo & 0xFFFFu), at(xb + (lo >> 16)), at(lo2 & 0xFFFFu), at(xb + (lo2 >> 16)),
+               at(xb + (hi >> 16))};
   };
-  for (int l = 0; l < 64; l++) load(0, l, ga[l], xbv[l], xi[l]);
+  for (int l = 0; l < 64; l++) cur[l] = load(0, l);
   for (int t = 0; t < n; t++) {
-    for (int l = 0; l < 64; l++) load(t + 1, l, gn[l], bn[l], in[l]);
-    const unsigned step_div = (unsigned)(prog[t * 64] >> 32) & 2u;
+    for (int l = 0; l < 64; l++) nxt[l] = load(t + 1, l);
+    const unsigned step_div = (unsigned)progh[t * 64] & 2u;
     for (int l = 0; l < 64; l++) {
-      const unsigned hi = (unsigned)(prog[t * 64 + l] >> 32);
-      if (((hi & 2u) != step_div) || ((hi & 1u) && !step_div)) abort();
+      const unsigned hi = (unsigned)progh[t * 64 + l];
+      if (((hi & 2u) != step_div) || ((hi & 5u) && !step_div)) abort();
       // the device's test of a chain's first step, and what it stands for
       const bool first = solve ? (hi >= 0x10000u) : (hi < (C::GZ << 16));
       if (first != ((hi >> 16) != noinit)) abort();
-      const double v0 = first ? xi[l] : acc[l];
-      const double v = (step_div && (hi & 1u)) ? v0 / ga[l] : v0 - ga[l] * xbv[l];
+      const Ops& o = cur[l];
+      const double v0 = first ? o.xi : acc[l];
+      const double v1 = (hi & 4u) ? v0 / o.ga : v0 - o.ga * o.xb;
+      const double v = (hi & 1u) ? v1 / o.ga2 : v1 - o.ga2 * o.xb2;
       at(xb + (hi & 0xFFF8u)) = v;
       acc[l] = v;
     }
-    for (int l = 0; l < 64; l++) { ga[l] = gn[l]; xbv[l] = bn[l]; xi[l] = in[l]; }
+    for (int l = 0; l < 64; l++) cur[l] = nxt[l];
   }
   zero_slot_intact();
 }
 extern "C" void coop_lu(double* G, double* X) {
   fresh();
   for (int n = 0; n < M::LU_NONZERO; n++) at(8 * C::GSLOT()[n]) = G[n];
-  program(C::LU(), C::LU_STEPS, false);
+  program(C::LU(), C::LUH(), C::LU_STEPS, false);
   for (int n = 0; n < M::LU_NONZERO; n++) G[n] = at(8 * C::GSLOT()[n]);
   for (int i = 0; i < M::NVAR; i++) at(C::GZ + 16 + 8 * i) = X[i];
-  program(C::SOL(), C::SOL_STEPS, true);
+  program(C::SOL(), C::SOLH(), C::SOL_STEPS, true);
   for (int i = 0; i < M::NVAR; i++) X[i] = at(C::GZ + 16 + 8 * (M::NVAR + i));
 }
 #else
