@@ -207,6 +207,145 @@ def rel_errors(X, ref, nvar, atol, rtol):
     return out
 
 
+def model_call_bytes(mech):
+    """Algorithmic HBM bytes of one call of each model routine on a cell's
+    tile (read and written once): Fun reads V, F, RCONST and writes Vdot;
+    Jac_SP reads V, F, RCONST and writes the LU_NONZERO entries; a
+    factorisation reads and writes the LU entries; a solve reads the LU
+    entries and reads and writes the right-hand side."""
+    nv, nf, nr, nnz = mech.nvar, mech.nfix, mech.nreact, mech.lu_nonzero
+    return {"fun": 8 * (2 * nv + nf + nr), "jac": 8 * (nv + nf + nr + nnz),
+            "dec": 8 * 2 * nnz, "sol": 8 * (nnz + 2 * nv)}
+
+
+def bench_integrator(args):
+    """One of the other integrators on the workload's cells: HIP-event time of
+    the integration (inputs resident, states reset between steps, untimed),
+    cells/s, the roofline from the per-cell counters (Nfun, Njac, Ndec, Nsol
+    times each routine's bytes; the complex factorisations and solves of the
+    Runge-Kutta integrator count twice), and the CPU oracle of the same
+    integrator on a bounded sample of the cells."""
+    import torch
+
+    from kpp_amd import solver, workloads
+    from oracle import ref
+
+    wl_fn = workloads.WORKLOADS[args.workload]
+    n = args.ncells
+    b = wl_fn(ncells=n, seed=args.seed)
+    mech = solver.mechanism(b.mech)
+    dev = torch.device("cuda:0")
+    d = b.to_device(dev)
+    C0 = d["C"].clone()
+    nv = mech.nvar
+    name = args.integrator
+    ic = np.zeros(20, np.int32)
+    rc = np.zeros(20)
+    lam0 = None
+    if name in ("f90", "h211b"):
+        ic[1], ic[14] = 1, 1  # scalar tolerances, rates at every call
+        if name == "h211b":
+            rc[14], rc[15] = 2.0, 3.0
+        label = {"f90": "F90 Rosenbrock (Rodas3)", "h211b": "F90 Rosenbrock with H211b"}[name]
+    elif name == "sdirk":
+        ic[1] = 1
+        label = "SDIRK (Sdirk4A)"
+    elif name == "rk":
+        ic[1], ic[9] = 1, 1
+        label = "Runge-Kutta (Radau-2A, SDIRK error estimate)"
+    else:
+        ic[1], ic[2] = 1, 4
+        label = "discrete adjoint Rosenbrock (Rodas3, 1 adjoint vector)"
+        lam0 = torch.ones((nv, n), dtype=torch.float64, device=dev)
+    nchk = 128  # checkpoint slots per cell (a cell needing more ends with -12)
+
+    def run():
+        d["C"].copy_(C0)
+        lam = None if lam0 is None else lam0.clone()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        if name in ("f90", "h211b"):
+            out = solver.integrate_f90(b.mech, d["C"], d["temp"], d["cfactor"], tin=d["tstart"],
+                                       tout=d["tend"], atol=b.atol, rtol=b.rtol, ICNTRL=ic,
+                                       RCNTRL=rc, sun=d["sun"], h211b=name == "h211b",
+                                       variant=solver.PERSISTENT)
+        elif name == "sdirk":
+            out = solver.sdirk(b.mech, d["C"], d["temp"], d["cfactor"], tstart=d["tstart"],
+                               tend=d["tend"], atol=b.atol, rtol=b.rtol, ICNTRL=ic, RCNTRL=rc,
+                               sun=d["sun"], sun_mode=b.sun_mode)
+        elif name == "rk":
+            out = solver.runge_kutta(b.mech, d["C"], d["temp"], d["cfactor"], tstart=d["tstart"],
+                                     tend=d["tend"], atol=b.atol, rtol=b.rtol, ICNTRL=ic,
+                                     RCNTRL=rc, sun=d["sun"], sun_mode=b.sun_mode)
+        else:
+            out = solver.rosenbrock_adj(b.mech, d["C"], d["temp"], d["cfactor"],
+                                        tstart=d["tstart"], tend=d["tend"], atol=b.atol,
+                                        rtol=b.rtol, Lambda=lam, ICNTRL=ic, RCNTRL=rc, nchk=nchk,
+                                        sun=d["sun"], sun_mode=b.sun_mode)
+        e1.record()
+        torch.cuda.synchronize()
+        code, ierr, istat, rstat = out
+        if code != 0:
+            raise SystemExit(f"{name} returned {code}")
+        return e0.elapsed_time(e1), ierr.cpu().numpy(), istat.cpu().numpy()
+
+    for _ in range(args.warmup):
+        run()
+    times = []
+    for _ in range(args.steps):
+        ms, ierr, istat = run()
+        times.append(ms)
+        log(f"[bench] {name}: {ms:.1f} ms")
+    ms = float(np.mean(times))
+    by = model_call_bytes(mech)
+    cplx = 2 if name == "rk" else 1
+    nbytes = float(istat[0].sum() * by["fun"] + istat[1].sum() * by["jac"] +
+                   istat[5].sum() * by["dec"] * cplx + istat[6].sum() * by["sol"] * cplx)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    # the CPU oracle of the same integrator on a bounded sample of the cells
+    cpu = None
+    if not args.no_cpu:
+        threads = host_cpu()["affinity_logical_cpus"]
+
+        def oracle(ns):
+            cells = np.linspace(0, n - 1, ns).astype(np.int64)
+            bs = wl_fn(seed=args.seed, cells=cells)
+            t = time.time()
+            if name in ("f90", "h211b"):
+                o = ref.integrate_f90(bs, ic, rc, nthreads=threads, fast=True,
+                                      h211b=name == "h211b")
+            elif name == "sdirk":
+                o = ref.integrate_sdirk(bs, ic, rc, nthreads=threads, fast=True)
+            elif name == "rk":
+                o = ref.integrate_rk(bs, ic, rc, nthreads=threads, fast=True)
+            else:
+                o = ref.integrate_adj(bs, np.ones((nv, ns)), ic, rc, cap=nchk, nthreads=threads,
+                                      fast=True)
+            return o, time.time() - t
+
+        # a calibration sample, then one sized to about 10 s of CPU work
+        o, dt = oracle(min(n, 2048))
+        ns = int(min(n, max(2048, 2048 * 10.0 / max(dt, 1e-3))))
+        o, dt = oracle(ns)
+        cpu = {"value": ns / dt, "unit": "cells/s", "cores": threads, "kind": "port",
+               "sample": f"{ns} evenly spaced cells of the same {n}-cell batch, the oracle's "
+                         f"{name} integrator (gcc -O2 -march=x86-64-v3 -fopenmp), {dt:.1f} s",
+               "ierr_ok": bool((o["ierr"] == 1).all())}
+    print(json.dumps({
+        "metric": f"cells/s integrated ({args.workload}): {label}",
+        "value": n / (ms * 1e-3), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "mechanism": b.mech, "ncells": n,
+                   "integrator": name, "all_ierr_1": bool((ierr == 1).all()),
+                   "mean_counters": [float(x) for x in istat.mean(axis=1)]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": nbytes},
+        "cpu_baseline": cpu}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,6 +374,10 @@ def main():
                     help="also time the mixed-precision integrator and report it as value")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--integrator", default="",
+                    choices=["", "f90", "h211b", "sdirk", "rk", "adj"],
+                    help="time one of the other integrators instead (F90 Rosenbrock, H211b, "
+                         "SDIRK, Runge-Kutta, discrete adjoint) on --workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): cells are "
@@ -243,6 +386,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.integrator:
+        bench_integrator(args)
         return
 
     import torch
