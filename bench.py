@@ -249,7 +249,7 @@ def bench_integrator(args):
         label = {"f90": "F90 Rosenbrock (Rodas3)", "h211b": "F90 Rosenbrock with H211b"}[name]
     elif name == "sdirk":
         ic[1] = 1
-        label = "SDIRK (Sdirk4A)"
+        label = "SDIRK (Sdirk2A, the default of ICNTRL[2] = 0)"
     elif name == "rk":
         ic[1], ic[9] = 1, 1
         label = "Runge-Kutta (Radau-2A, SDIRK error estimate)"
