@@ -10,6 +10,30 @@
 #include "sdirk_device.h"
 
 namespace kpphip {
+// the resident grid of a persistent kernel: as many one-wavefront blocks as
+// the device holds at once, at most one per 64 cells
+template <class K>
+int resident_blocks(K kernel, int nblocks) {
+  int occ = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, TILE, 0) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return nblocks;
+  const int nb = (occ > 0 ? occ : 1) * (cus > 0 ? cus : 1);
+  return nb < nblocks ? nb : nblocks;
+}
+// one-cell-per-lane grid, or (p.queue set, zero on the stream) the
+// persistent one
+template <class K1, class K2>
+hipError_t launch_plain_or_queue(K1 plain, K2 persistent, const RosParams& p, int nblocks,
+                                 hipStream_t st) {
+  if (p.queue)
+    hipLaunchKernelGGL(persistent, dim3(resident_blocks(persistent, nblocks)), dim3(TILE), 0, st,
+                       p);
+  else
+    hipLaunchKernelGGL(plain, dim3(nblocks), dim3(TILE), 0, st, p);
+  return hipGetLastError();
+}
 // SDIRK kernels, built for mechanisms whose LU fits a plain tile of modest
 // size (not for the largest ones, whose model routines are split into parts)
 template <class M>
@@ -31,12 +55,17 @@ template <class M>
 hipError_t sdirk_launch_mech(const RosParams& p, int nblocks, hipStream_t st) {
   if constexpr (has_sdirk<M>()) {
     switch (p.sd.S) {
-      case 2: hipLaunchKernelGGL((sdirk_plain_kernel<M, 2>), dim3(nblocks), dim3(TILE), 0, st, p); break;
-      case 3: hipLaunchKernelGGL((sdirk_plain_kernel<M, 3>), dim3(nblocks), dim3(TILE), 0, st, p); break;
-      case 5: hipLaunchKernelGGL((sdirk_plain_kernel<M, 5>), dim3(nblocks), dim3(TILE), 0, st, p); break;
+      case 2:
+        return launch_plain_or_queue(sdirk_plain_kernel<M, 2>, sdirk_persistent_kernel<M, 2>, p,
+                                     nblocks, st);
+      case 3:
+        return launch_plain_or_queue(sdirk_plain_kernel<M, 3>, sdirk_persistent_kernel<M, 3>, p,
+                                     nblocks, st);
+      case 5:
+        return launch_plain_or_queue(sdirk_plain_kernel<M, 5>, sdirk_persistent_kernel<M, 5>, p,
+                                     nblocks, st);
       default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
   }
   return hipErrorInvalidValue;
 }
@@ -51,10 +80,8 @@ int rk_rows_mech(int) {
 }
 template <class M>
 hipError_t rk_launch_mech(const RosParams& p, int nblocks, hipStream_t st) {
-  if constexpr (has_sdirk<M>()) {
-    hipLaunchKernelGGL((rk_plain_kernel<M>), dim3(nblocks), dim3(TILE), 0, st, p);
-    return hipGetLastError();
-  }
+  if constexpr (has_sdirk<M>())
+    return launch_plain_or_queue(rk_plain_kernel<M>, rk_persistent_kernel<M>, p, nblocks, st);
   return hipErrorInvalidValue;
 }
 }  // namespace kpphip

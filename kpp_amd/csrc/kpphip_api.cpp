@@ -913,6 +913,11 @@ int implicit_batch(const MechOps* m, const kpp_batch* b, bool vector_tol, const 
   p.order = b->order;
   p.lanes = TILE_LANES;
   p.scratch = (double*)(ws + kHead + align256(3 * (size_t)m->nvar * sizeof(double)));
+  // the work queue of the persistent kernels (workspace head), zeroed by a
+  // kernel on the stream
+  p.queue = (int*)ws;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, p.queue, 1, 0);
+  if (hipGetLastError() != hipSuccess) return KPP_E_HIP;
   return 0;
 }
 

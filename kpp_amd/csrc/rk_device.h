@@ -26,6 +26,7 @@
 // or non-finite step ends it with -14, and more than KPP_MAX_REJECT passes
 // in a row in which T does not advance with -15.
 #pragma once
+#include "refill.h"
 #include "ros_device.h"
 
 #define KPP_RK_IERR_NONFINITE (-14)
@@ -496,6 +497,14 @@ __global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void rk_plain_kernel(RosPara
   c.start(p, p.order ? p.order[k] : k, s);
   while (c.attempt(p, s) == 0) {
   }
+}
+
+// A resident grid whose lanes take cells from a work queue (refill_loop).
+template <class M>
+__global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void rk_persistent_kernel(RosParams p) {
+  using L = RkLayout<M>;
+  const RkCell<M> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + threadIdx.x};
+  refill_loop<RkCell<M>, RkState>(p, c);
 }
 
 }  // namespace kpphip

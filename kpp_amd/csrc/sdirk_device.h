@@ -22,6 +22,7 @@
 // ends the cell with -9, more than KPP_MAX_REJECT failed passes in a row
 // with -10.
 #pragma once
+#include "refill.h"
 #include "ros_device.h"
 
 namespace kpphip {
@@ -354,6 +355,14 @@ __global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void sdirk_plain_kernel(RosP
   c.start(p, p.order ? p.order[k] : k, s);
   while (c.attempt(p, s) == 0) {
   }
+}
+
+// A resident grid whose lanes take cells from a work queue (refill_loop).
+template <class M, int S>
+__global__ __launch_bounds__(64) KPPHIP_KERNEL_ATTR void sdirk_persistent_kernel(RosParams p) {
+  using L = SdLayout<M, S>;
+  const SdCell<M, S> c{p.scratch + (size_t)blockIdx.x * L::ROWS * TILE + threadIdx.x};
+  refill_loop<SdCell<M, S>, SdState>(p, c);
 }
 
 }  // namespace kpphip
