@@ -10,30 +10,6 @@
 #include "sdirk_device.h"
 
 namespace kpphip {
-// the resident grid of a persistent kernel: as many one-wavefront blocks as
-// the device holds at once, at most one per 64 cells
-template <class K>
-int resident_blocks(K kernel, int nblocks) {
-  int occ = 0, dev = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, TILE, 0) != hipSuccess ||
-      hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    return nblocks;
-  const int nb = (occ > 0 ? occ : 1) * (cus > 0 ? cus : 1);
-  return nb < nblocks ? nb : nblocks;
-}
-// one-cell-per-lane grid, or (p.queue set, zero on the stream) the
-// persistent one
-template <class K1, class K2>
-hipError_t launch_plain_or_queue(K1 plain, K2 persistent, const RosParams& p, int nblocks,
-                                 hipStream_t st) {
-  if (p.queue)
-    hipLaunchKernelGGL(persistent, dim3(resident_blocks(persistent, nblocks)), dim3(TILE), 0, st,
-                       p);
-  else
-    hipLaunchKernelGGL(plain, dim3(nblocks), dim3(TILE), 0, st, p);
-  return hipGetLastError();
-}
 // SDIRK kernels, built for mechanisms whose LU fits a plain tile of modest
 // size (not for the largest ones, whose model routines are split into parts)
 template <class M>

@@ -1,6 +1,8 @@
-// Work queue of the implicit integrators' persistent kernels (SDIRK,
-// Runge-Kutta), included by their device headers.
+// Work queue of the persistent kernels of the SDIRK and Runge-Kutta
+// integrators, and their launch, included by their device headers.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include "ros_device.h"
 
 namespace kpphip {
@@ -39,4 +41,28 @@ __device__ __forceinline__ void refill_loop(const RosParams& p, const CellT& c) 
   }
 }
 
+// the resident grid of a persistent kernel: as many one-wavefront blocks as
+// the device holds at once, at most one per 64 cells
+template <class K>
+int resident_blocks(K kernel, int nblocks) {
+  int occ = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, TILE, 0) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return nblocks;
+  const int nb = (occ > 0 ? occ : 1) * (cus > 0 ? cus : 1);
+  return nb < nblocks ? nb : nblocks;
+}
+// one-cell-per-lane grid, or (p.queue set, zero on the stream) the
+// persistent one
+template <class K1, class K2>
+hipError_t launch_plain_or_queue(K1 plain, K2 persistent, const RosParams& p, int nblocks,
+                                 hipStream_t st) {
+  if (p.queue)
+    hipLaunchKernelGGL(persistent, dim3(resident_blocks(persistent, nblocks)), dim3(TILE), 0, st,
+                       p);
+  else
+    hipLaunchKernelGGL(plain, dim3(nblocks), dim3(TILE), 0, st, p);
+  return hipGetLastError();
+}
 }  // namespace kpphip
