@@ -258,6 +258,11 @@ def bench_integrator(args):
         label = "discrete adjoint Rosenbrock (Rodas3, 1 adjoint vector)"
         lam0 = torch.ones((nv, n), dtype=torch.float64, device=dev)
     nchk = 128  # checkpoint slots per cell (a cell needing more ends with -12)
+    model = None
+    if args.learned_order and name in ("sdirk", "rk"):
+        from kpp_amd import learned
+        model = learned.StepModel.load(args.weights, dev)
+        span = float(np.max(np.abs(b.tend - b.tstart)))
 
     def run():
         d["C"].copy_(C0)
@@ -265,6 +270,9 @@ def bench_integrator(args):
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
+        order = None
+        if model is not None:
+            order = model.schedule(model.features(d["C"], d["temp"], d["sun"]), 1e-6, span)[2]
         if name in ("f90", "h211b"):
             out = solver.integrate_f90(b.mech, d["C"], d["temp"], d["cfactor"], tin=d["tstart"],
                                        tout=d["tend"], atol=b.atol, rtol=b.rtol, ICNTRL=ic,
@@ -273,11 +281,11 @@ def bench_integrator(args):
         elif name == "sdirk":
             out = solver.sdirk(b.mech, d["C"], d["temp"], d["cfactor"], tstart=d["tstart"],
                                tend=d["tend"], atol=b.atol, rtol=b.rtol, ICNTRL=ic, RCNTRL=rc,
-                               sun=d["sun"], sun_mode=b.sun_mode)
+                               sun=d["sun"], sun_mode=b.sun_mode, order=order)
         elif name == "rk":
             out = solver.runge_kutta(b.mech, d["C"], d["temp"], d["cfactor"], tstart=d["tstart"],
                                      tend=d["tend"], atol=b.atol, rtol=b.rtol, ICNTRL=ic,
-                                     RCNTRL=rc, sun=d["sun"], sun_mode=b.sun_mode)
+                                     RCNTRL=rc, sun=d["sun"], sun_mode=b.sun_mode, order=order)
         else:
             out = solver.rosenbrock_adj(b.mech, d["C"], d["temp"], d["cfactor"],
                                         tstart=d["tstart"], tend=d["tend"], atol=b.atol,
@@ -338,7 +346,8 @@ def bench_integrator(args):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": args.workload, "mechanism": b.mech, "ncells": n,
-                   "integrator": name, "all_ierr_1": bool((ierr == 1).all()),
+                   "integrator": name, "learned_order": model is not None,
+                   "all_ierr_1": bool((ierr == 1).all()),
                    "mean_counters": [float(x) for x in istat.mean(axis=1)]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -374,6 +383,9 @@ def main():
                     help="also time the mixed-precision integrator and report it as value")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--learned-order", action="store_true",
+                    help="with --integrator sdirk/rk: launch cells in the order the step "
+                         "scheduler predicts (most steps first), inside the timed region")
     ap.add_argument("--integrator", default="",
                     choices=["", "f90", "h211b", "sdirk", "rk", "adj"],
                     help="time one of the other integrators instead (F90 Rosenbrock, H211b, "

@@ -386,7 +386,8 @@ def integrate_f90(mech: str, C, temp, cfactor, *, tin, tout, atol, rtol, ICNTRL=
 
 
 def _implicit(which, mech, C, temp, cfactor, tstart, tend, atol, rtol, ICNTRL, RCNTRL, sun,
-              sun_mode, hstart, rconst, ierr, istat, rstat, workspace, stream, adj=None):
+              sun_mode, hstart, rconst, ierr, istat, rstat, workspace, stream, adj=None,
+              order=None):
     """kpp_sdirk_dev / kpp_rk_dev / kpp_rosenbrock_adj_dev on device tensors
     (the shared batch set-up); adj = (nadj, Lambda, nchk, first_step)"""
     import torch
@@ -419,6 +420,10 @@ def _implicit(which, mech, C, temp, cfactor, tstart, tend, atol, rtol, ICNTRL, R
     b.ierr = ierr.data_ptr()
     b.istat = istat.data_ptr()
     b.rstat = rstat.data_ptr()
+    if order is not None:
+        if order.dtype != torch.int32 or order.numel() != n:
+            raise KppError("order must be an int32 tensor of ncells cell indices")
+        b.order = order.data_ptr()
     ic = np.zeros(20, np.int32) if ICNTRL is None else np.ascontiguousarray(ICNTRL, np.int32)
     rc = np.zeros(20) if RCNTRL is None else np.ascontiguousarray(RCNTRL, np.float64)
     atol = np.ascontiguousarray(np.atleast_1d(atol), np.float64)
@@ -453,24 +458,26 @@ def _implicit(which, mech, C, temp, cfactor, tstart, tend, atol, rtol, ICNTRL, R
 
 def sdirk(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, ICNTRL=None, RCNTRL=None,
           sun=None, sun_mode=SUN_FIXED, hstart=None, rconst=None, ierr=None, istat=None,
-          rstat=None, workspace=None, stream=None):
+          rstat=None, workspace=None, stream=None, order=None):
     """KPP's C SDIRK (int/sdirk.c) on device tensors, one lane per cell.
     ICNTRL/RCNTRL: 20 entries as SDIRK takes them (None = zeros).  Returns
     (code, ierr, istat [8, n] = ISTATUS[1..8], rstat [3, n] = RSTATUS[1..3])."""
     return _implicit("sdirk", mech, C, temp, cfactor, tstart, tend, atol, rtol, ICNTRL, RCNTRL,
-                     sun, sun_mode, hstart, rconst, ierr, istat, rstat, workspace, stream)
+                     sun, sun_mode, hstart, rconst, ierr, istat, rstat, workspace, stream,
+                     order=order)
 
 
 def runge_kutta(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, ICNTRL=None,
                 RCNTRL=None, sun=None, sun_mode=SUN_FIXED, hstart=None, rconst=None, ierr=None,
-                istat=None, rstat=None, workspace=None, stream=None):
+                istat=None, rstat=None, workspace=None, stream=None, order=None):
     """KPP's C fully implicit Runge-Kutta (int/runge_kutta.c) on device
     tensors, one lane per cell.  ICNTRL/RCNTRL: 20 entries as RungeKutta
     takes them (None = zeros: Radau-2A, classical error estimate,
     Gustafsson's controller).  Returns (code, ierr, istat [8, n] =
     ISTATUS[0..7], rstat [3, n] = RSTATUS[0..2])."""
     return _implicit("rk", mech, C, temp, cfactor, tstart, tend, atol, rtol, ICNTRL, RCNTRL,
-                     sun, sun_mode, hstart, rconst, ierr, istat, rstat, workspace, stream)
+                     sun, sun_mode, hstart, rconst, ierr, istat, rstat, workspace, stream,
+                     order=order)
 
 
 def rosenbrock_adj(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, Lambda,
