@@ -259,7 +259,7 @@ def bench_integrator(args):
         lam0 = torch.ones((nv, n), dtype=torch.float64, device=dev)
     nchk = 128  # checkpoint slots per cell (a cell needing more ends with -12)
     model = None
-    if args.learned_order and name in ("sdirk", "rk"):
+    if args.learned_order and name in ("sdirk", "rk", "adj"):
         from kpp_amd import learned
         model = learned.StepModel.load(args.weights, dev)
         span = float(np.max(np.abs(b.tend - b.tstart)))
@@ -290,7 +290,7 @@ def bench_integrator(args):
             out = solver.rosenbrock_adj(b.mech, d["C"], d["temp"], d["cfactor"],
                                         tstart=d["tstart"], tend=d["tend"], atol=b.atol,
                                         rtol=b.rtol, Lambda=lam, ICNTRL=ic, RCNTRL=rc, nchk=nchk,
-                                        sun=d["sun"], sun_mode=b.sun_mode)
+                                        sun=d["sun"], sun_mode=b.sun_mode, order=order)
         e1.record()
         torch.cuda.synchronize()
         code, ierr, istat, rstat = out
@@ -384,7 +384,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--learned-order", action="store_true",
-                    help="with --integrator sdirk/rk: launch cells in the order the step "
+                    help="with --integrator sdirk/rk/adj: launch cells in the order the step "
                          "scheduler predicts (most steps first), inside the timed region")
     ap.add_argument("--integrator", default="",
                     choices=["", "f90", "h211b", "sdirk", "rk", "adj"],

@@ -483,7 +483,7 @@ def runge_kutta(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, ICNTRL
 def rosenbrock_adj(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, Lambda,
                    ICNTRL=None, RCNTRL=None, nchk=4096, first_step=False, sun=None,
                    sun_mode=SUN_FIXED, rconst=None, ierr=None, istat=None, rstat=None,
-                   workspace=None, stream=None):
+                   workspace=None, stream=None, order=None):
     """KPP's C discrete adjoint Rosenbrock (int/rosenbrock_adj.c) on device
     tensors, one lane per cell.  Lambda: float64 [NADJ*NVAR, ncells] (row
     a*NVAR + i = Lambda[a][i]), dg/dy at tend on entry, at tstart on return;
@@ -499,7 +499,7 @@ def rosenbrock_adj(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, Lam
             raise KppError("Lambda must be a contiguous float64 tensor [NADJ*NVAR, ncells]")
     return _implicit("adj", mech, C, temp, cfactor, tstart, tend, atol, rtol, ICNTRL, RCNTRL,
                      sun, sun_mode, None, rconst, ierr, istat, rstat, workspace, stream,
-                     adj=(nadj, Lambda, int(nchk), bool(first_step)))
+                     adj=(nadj, Lambda, int(nchk), bool(first_step)), order=order)
 
 
 def _comp(fn, *args):

@@ -192,3 +192,25 @@ def test_errors_limits_and_unsupported(gpu):
     bm = workloads.mcm_mixed(ncells=4)
     with pytest.raises(solver.KppError):
         _gpu(bm, gpu, None, _ic(4))
+
+
+def test_launch_order_leaves_results_unchanged(gpu):
+    """Cells launched in a permuted order (lane k integrates cell order[k])
+    give bit-identical states, Lambda and counters."""
+    import torch
+    b = workloads.small_strato_tod(ncells=200)
+    nv = ref.sizes(b.mech)["NVAR"]
+    perm = torch.from_numpy(np.random.default_rng(4).permutation(200).astype(np.int32)).to(gpu)
+    outs = []
+    for order in (None, perm):
+        d = b.to_device(gpu)
+        lam = torch.ones((nv, 200), dtype=torch.float64, device=gpu)
+        code, ierr, istat, rstat = solver.rosenbrock_adj(
+            b.mech, d["C"], d["temp"], d["cfactor"], tstart=d["tstart"], tend=d["tend"],
+            atol=b.atol, rtol=b.rtol, Lambda=lam, ICNTRL=_ic(4), sun=d["sun"],
+            sun_mode=b.sun_mode, order=order)
+        torch.cuda.synchronize()
+        assert code == 0 and bool((ierr == 1).all())
+        outs.append((d["C"].cpu().numpy(), lam.cpu().numpy(), istat.cpu().numpy()))
+    for x, y in zip(*outs):
+        assert np.array_equal(x, y)
