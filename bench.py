@@ -259,7 +259,7 @@ def bench_integrator(args):
         lam0 = torch.ones((nv, n), dtype=torch.float64, device=dev)
     nchk = 128  # checkpoint slots per cell (a cell needing more ends with -12)
     model = None
-    if args.learned_order and name in ("sdirk", "rk", "adj"):
+    if args.learned_order:
         from kpp_amd import learned
         model = learned.StepModel.load(args.weights, dev)
         span = float(np.max(np.abs(b.tend - b.tstart)))
@@ -277,7 +277,7 @@ def bench_integrator(args):
             out = solver.integrate_f90(b.mech, d["C"], d["temp"], d["cfactor"], tin=d["tstart"],
                                        tout=d["tend"], atol=b.atol, rtol=b.rtol, ICNTRL=ic,
                                        RCNTRL=rc, sun=d["sun"], h211b=name == "h211b",
-                                       variant=solver.PERSISTENT)
+                                       variant=solver.PERSISTENT, order=order)
         elif name == "sdirk":
             out = solver.sdirk(b.mech, d["C"], d["temp"], d["cfactor"], tstart=d["tstart"],
                                tend=d["tend"], atol=b.atol, rtol=b.rtol, ICNTRL=ic, RCNTRL=rc,
@@ -384,7 +384,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--learned-order", action="store_true",
-                    help="with --integrator sdirk/rk/adj: launch cells in the order the step "
+                    help="with --integrator: launch cells in the order the step "
                          "scheduler predicts (most steps first), inside the timed region")
     ap.add_argument("--integrator", default="",
                     choices=["", "f90", "h211b", "sdirk", "rk", "adj"],

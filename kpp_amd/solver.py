@@ -313,7 +313,7 @@ def rosenbrock(mech: str, C, temp, cfactor, *, tstart, tend, atol, rtol, IPAR, R
 def integrate_f90(mech: str, C, temp, cfactor, *, tin, tout, atol, rtol, ICNTRL=None,
                   RCNTRL=None, sun=None, passive=None, hstart=None, rconst=None, ierr=None,
                   istat=None, rstat=None, variant=PLAIN, nblocks=0, lanes=0, workspace=None,
-                  stream=None, h211b=False):
+                  stream=None, h211b=False, order=None):
     """The F90 INTEGRATE (int/rosenbrock.f90:43-125) on device tensors, one
     call per cell over [tin, tout] (per-cell tensors or floats).
 
@@ -356,6 +356,10 @@ def integrate_f90(mech: str, C, temp, cfactor, *, tin, tout, atol, rtol, ICNTRL=
     b.ierr = ierr.data_ptr()
     b.istat = istat.data_ptr()
     b.rstat = rstat.data_ptr()
+    if order is not None:  # launch order (cell permutation); results per cell unchanged
+        if order.dtype != torch.int32 or order.numel() != n:
+            raise KppError("order must be an int32 tensor of ncells cell indices")
+        b.order = order.data_ptr()
     ic = None if ICNTRL is None else np.ascontiguousarray(ICNTRL, np.int32)
     rc = None if RCNTRL is None else np.ascontiguousarray(RCNTRL, np.float64)
     pas = None if passive is None else np.ascontiguousarray(passive, np.int32)
