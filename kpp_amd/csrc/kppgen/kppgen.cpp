@@ -2844,24 +2844,28 @@ uint64_t op_desc(int a, int b, int t, bool first, bool last, bool div) {
 // acc + (-0.0)*0) and writes it back to the lane's previous target, or JUNK.
 constexpr uint32_t T_JUNK = 0, T_ZERO = 8, T_DATA = 16;
 
-// Sum programs: lo = product offset (T) | target offset << 16 (Fun: T, its
-// results region at fo; Jac_SP: G); hi bit 31 = FIRST.  Coefficient per
-// step: the chain's, 1.0 for the single step of an empty chain (0.0 * 1.0),
-// -0.0 for idle steps.
-void encode_sums(const std::vector<uint64_t>& tab, bool jvs, uint32_t gz, uint32_t fo,
-                 const std::vector<double>& pool, const std::vector<int>& gslot,
-                 const std::vector<int>& pslot, std::vector<uint64_t>& out,
-                 std::vector<double>& coef) {
+// Sum programs, two terms of a chain per step: descriptor = product offset
+// (T) | target offset << 16 (Fun: T, its results region at fo; Jac_SP: G) |
+// second product offset << 32; bit 63 = FIRST.  Coefficients per step and
+// term (two tables): the chain's, 1.0 for the single term of an empty chain
+// (0.0 * 1.0), -0.0 for an absent term (its product is the +0.0 slot) and
+// for idle steps.
+void encode_sums(const std::vector<uint64_t>& tab, const std::vector<uint64_t>& tab2, bool jvs,
+                 uint32_t gz, uint32_t fo, const std::vector<double>& pool,
+                 const std::vector<int>& gslot, const std::vector<int>& pslot,
+                 std::vector<uint64_t>& out, std::vector<double>& coef,
+                 std::vector<double>& coef2) {
   const size_t steps = tab.size() / 64;
   out.assign(tab.size(), 0);
   coef.assign(tab.size(), 0.0);
+  coef2.assign(tab.size(), 0.0);
   const uint32_t junk = jvs ? gz + T_JUNK : T_JUNK;
   for (int l = 0; l < 64; l++) {
     uint32_t prev = junk;
     for (size_t t = 0; t < steps; t++) {
-      const uint64_t d = tab[t * 64 + l];
-      uint32_t p = T_ZERO, s = prev, first = 0;
-      double c = -0.0;
+      const uint64_t d = tab[t * 64 + l], d2 = tab2[t * 64 + l];
+      uint32_t p = T_ZERO, p2 = T_ZERO, s = prev, first = 0;
+      double c = -0.0, c2 = -0.0;
       if (d & VALID) {
         const bool zero = (d >> 50) & 1;
         const uint32_t src = (uint32_t)((d >> 16) & 0xFFFF), tgt = (uint32_t)((d >> 32) & 0xFFFF);
@@ -2870,9 +2874,17 @@ void encode_sums(const std::vector<uint64_t>& tab, bool jvs, uint32_t gz, uint32
         s = jvs ? 8 * (uint32_t)gslot[tgt] : fo + 8 * tgt;
         first = (d >> 48) & 1;
         prev = s;
+        if (d2 & VALID) {
+          if (((d2 >> 32) & 0xFFFF) != tgt || ((d2 >> 48) & 1) || ((d2 >> 50) & 1))
+            die("coop: a pair spans two chains");
+          p2 = T_DATA + 8 * (uint32_t)pslot[(d2 >> 16) & 0xFFFF];
+          c2 = pool[d2 & 0xFFFF];
+        }
       }
-      out[t * 64 + l] = (uint64_t)p | ((uint64_t)s << 16) | ((uint64_t)first << 63);
+      out[t * 64 + l] = (uint64_t)p | ((uint64_t)s << 16) | ((uint64_t)p2 << 32) |
+                        ((uint64_t)first << 63);
       coef[t * 64 + l] = c;
+      coef2[t * 64 + l] = c2;
     }
   }
 }
@@ -3166,11 +3178,11 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
     return;
   }
   int tf = 0, tj = 0, tl = 0, ts = 0;
-  for (auto& c : lu) c = pair_chain(c);
-  for (auto& c : sol) c = pair_chain(c);
-  std::vector<uint64_t> tlu2, tsol2;
-  auto tfun = schedule(fun, 2, tf), tjac = schedule(jac, 2, tj), tlu = schedule(lu, 2, tl, &tlu2),
-       tsol = schedule(sol, 2, ts, &tsol2);
+  for (auto* v : {&fun, &jac, &lu, &sol})
+    for (auto& c : *v) c = pair_chain(c);
+  std::vector<uint64_t> tfun2, tjac2, tlu2, tsol2;
+  auto tfun = schedule(fun, 2, tf, &tfun2), tjac = schedule(jac, 2, tj, &tjac2),
+       tlu = schedule(lu, 2, tl, &tlu2), tsol = schedule(sol, 2, ts, &tsol2);
   auto hex = [](uint64_t v) {
     char b[32];
     snprintf(b, sizeof b, "0x%llxull", (unsigned long long)v);
@@ -3189,7 +3201,8 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   // LU entries' LDS slots: the LU program reads fields a, b and its
   // targets (bits 0-15, 16-31, 32-47), the solve field a, Jac_SP writes targets
   const std::vector<int> gslot =
-      g_coop_place ? place_lds(nnz, {&tlu, &tlu2, &tsol, &tsol2, &tjac}, {7, 3, 1, 1, 4}) : [&] {
+      g_coop_place ? place_lds(nnz, {&tlu, &tlu2, &tsol, &tsol2, &tjac}, {7, 3, 1, 1, 4})
+                   : [&] {
         std::vector<int> v(nnz);
         for (int n = 0; n < nnz; n++) v[n] = n;
         return v;
@@ -3232,8 +3245,8 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
     for (int k = 0; k < n; k++) v[k] = k;
     return v;
   };
-  const std::vector<int> aslot = g_coop_place ? place_lds(na, {&tfun}, {2}) : ident_of(na);
-  const std::vector<int> bslot = g_coop_place ? place_lds(nb, {&tjac}, {2}) : ident_of(nb);
+  const std::vector<int> aslot = g_coop_place ? place_lds(na, {&tfun, &tfun2}, {2, 2}) : ident_of(na);
+  const std::vector<int> bslot = g_coop_place ? place_lds(nb, {&tjac, &tjac2}, {2, 2}) : ident_of(nb);
   {
     std::vector<std::vector<uint32_t>> a2(na), b2(nb);
     for (int q = 0; q < na; q++) a2[aslot[q]] = aprod[q];
@@ -3242,9 +3255,9 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
     bprod.swap(b2);
   }
   std::vector<uint64_t> efun, ejac, elu, esol;
-  std::vector<double> cfun, cjac;
-  encode_sums(tfun, false, gz, fo, pool, gslot, aslot, efun, cfun);
-  encode_sums(tjac, true, gz, fo, pool, gslot, bslot, ejac, cjac);
+  std::vector<double> cfun, cjac, cfun2, cjac2;
+  encode_sums(tfun, tfun2, false, gz, fo, pool, gslot, aslot, efun, cfun, cfun2);
+  encode_sums(tjac, tjac2, true, gz, fo, pool, gslot, bslot, ejac, cjac, cjac2);
   std::vector<uint64_t> eluh, esolh;
   encode_ops(tlu, tlu2, false, gz, nv, gslot, elu, eluh);
   encode_ops(tsol, tsol2, true, gz, nv, gslot, esol, esolh);
@@ -3269,6 +3282,8 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   emit_table(o, "unsigned long long", "JAC", ejac.size(), [&](size_t i) { return hex(ejac[i]); });
   emit_table(o, "double", "FUN_COEF", cfun.size(), [&](size_t i) { return dbl(cfun[i]); });
   emit_table(o, "double", "JAC_COEF", cjac.size(), [&](size_t i) { return dbl(cjac[i]); });
+  emit_table(o, "double", "FUN_COEF2", cfun2.size(), [&](size_t i) { return dbl(cfun2[i]); });
+  emit_table(o, "double", "JAC_COEF2", cjac2.size(), [&](size_t i) { return dbl(cjac2[i]); });
   emit_table(o, "unsigned long long", "LU", elu.size(), [&](size_t i) { return hex(elu[i]); });
   emit_table(o, "unsigned long long", "SOL", esol.size(), [&](size_t i) { return hex(esol[i]); });
   emit_table(o, "unsigned long long", "LUH", eluh.size(), [&](size_t i) { return hex(eluh[i]); });
@@ -3289,12 +3304,13 @@ void emit_coop(Model& md, std::ostream& o, const std::string& ns) {
   o << "static constexpr int FUN_STEPS = " << tf << ", JAC_STEPS = " << tj
     << ", LU_STEPS = " << tl << ", SOL_STEPS = " << ts << ";\n";
   const char* names[] = {"APROD", "BPROD", "CONSTS", "FUN", "JAC", "LU", "SOL", "DIAG",
-                         "FUN_COEF", "JAC_COEF", "GSLOT", "LUH", "SOLH"};
+                         "FUN_COEF", "JAC_COEF", "GSLOT", "LUH", "SOLH", "FUN_COEF2",
+                         "JAC_COEF2"};
   const char* types[] = {"unsigned", "unsigned", "double", "unsigned long long",
                          "unsigned long long", "unsigned long long", "unsigned long long",
                          "unsigned", "double", "double", "unsigned", "unsigned long long",
-                         "unsigned long long"};
-  for (int k = 0; k < 13; k++)
+                         "unsigned long long", "double", "double"};
+  for (int k = 0; k < 15; k++)
     o << "static __device__ __forceinline__ const " << types[k] << "* " << names[k]
       << "() { return " << ns << "_coop_data::" << names[k] << "; }\n";
   o << "};\n";

@@ -141,52 +141,65 @@ __device__ __noinline__ void coop_products(const unsigned* Pg, int N, gcdouble* 
 __device__ __forceinline__ double lds_ld(unsigned addr) { return *(ldouble*)(size_t)addr; }
 __device__ __forceinline__ void lds_st(unsigned addr, double v) { *(ldouble*)(size_t)addr = v; }
 
-// A sum program (Vdot or JVS chains over the products in T).  Descriptor:
-// lo = product offset (T) | target offset << 16 (Fun: T, Jac_SP: G), hi bit
-// 31 = the chain's first step; per-step coefficient table.  Step:
-// term = coef * T[p]; acc = first ? term : acc + term; the target gets acc
-// (Jac_SP: -acc, the Ghimj build; the diagonal's +1/(h gamma) is added
-// afterwards).  Products are not written here, so their reads run two steps
-// ahead of use.
+// A sum program (Vdot or JVS chains over the products in T), two terms of a
+// chain per step.  Descriptor: product offset (T) | target offset << 16
+// (Fun: T, Jac_SP: G) | second product offset << 32; bit 63 = the chain's
+// first step; per-step coefficient tables for both terms.  Step: v = first ?
+// c1*T[p1] : acc + c1*T[p1]; v = v + c2*T[p2] (an absent term is -0.0 *
+// +0.0); the target gets v (Jac_SP: -v, the Ghimj build; the diagonal's
+// +1/(h gamma) is added afterwards).  Products are not written here, so their
+// reads run two steps ahead of use.
 template <bool JVS, unsigned GZ>
 __device__ __noinline__ void coop_sums(const unsigned long long* progg, const double* coefg,
-                                       int steps, int lane) {
+                                       const double* coef2g, int steps, int lane) {
   gcu64* prog = (gcu64*)progg;
   gcdouble* cf = (gcdouble*)coefg;
+  gcdouble* cf2 = (gcdouble*)coef2g;
   constexpr unsigned SB = JVS ? 0u : GZ;  // base of the target field
   unsigned long long r[kRing];
-  double c[kRing];
+  double c[kRing], c2[kRing];
   gcu64* q = prog + lane;
   gcdouble* qc = cf + lane;
+  gcdouble* qc2 = cf2 + lane;
 #pragma unroll
   for (int k = 0; k < kRing; k++) {
     r[k] = q[k * TILE];
     c[k] = qc[k * TILE];
+    c2[k] = qc2[k * TILE];
   }
   q += kRing * TILE;
   qc += kRing * TILE;
-  double x0 = lds_ld(GZ + ((unsigned)r[0] & 0xFFFFu)), x1 = lds_ld(GZ + ((unsigned)r[1] & 0xFFFFu));
+  qc2 += kRing * TILE;
+  auto p1 = [](unsigned long long d) { return lds_ld(GZ + ((unsigned)d & 0xFFFFu)); };
+  auto p2 = [](unsigned long long d) { return lds_ld(GZ + ((unsigned)(d >> 32) & 0xFFFFu)); };
+  double x0 = p1(r[0]), x1 = p1(r[1]), y0 = p2(r[0]), y1 = p2(r[1]);
   double acc = 0.0;
   for (int base = 0; base < steps; base += kRing) {
 #pragma unroll
     for (int k = 0; k < kRing; k++) {
       const unsigned long long d = r[k];
-      // product of step base+k+2 (slot k+2; slots 0 and 1 hold the next
+      // products of step base+k+2 (slot k+2; slots 0 and 1 hold the next
       // round's first two steps, refilled at k = 0 and 1)
-      const double x2 = lds_ld(GZ + ((unsigned)r[(k + 2) % kRing] & 0xFFFFu));
+      const double x2 = p1(r[(k + 2) % kRing]), y2 = p2(r[(k + 2) % kRing]);
       const double term = c[k] * x0;
       const double sum = acc + term;
-      acc = ((int)(unsigned)(d >> 32) < 0) ? term : sum;
+      const double v1 = ((long long)d < 0) ? term : sum;
+      const double term2 = c2[k] * y0;
+      acc = v1 + term2;
       lds_st(SB + ((unsigned)d >> 16), JVS ? -acc : acc);
       x0 = x1;
       x1 = x2;
+      y0 = y1;
+      y1 = y2;
       // refilled after the last use (see coop_program)
       __builtin_amdgcn_sched_barrier(0);
       r[k] = q[k * TILE];
       c[k] = qc[k * TILE];
+      c2[k] = qc2[k * TILE];
     }
     q += kRing * TILE;
     qc += kRing * TILE;
+    qc2 += kRing * TILE;
   }
 }
 
@@ -303,7 +316,7 @@ struct CoopCell {
     coop_products<C::MAXF, C::NA_PAD>(C::APROD(), C::NA, V, (gcdouble*)row(L::RCT),
                                       (gcdouble*)row(L::FIX), cst(), T, lane);
     sync();
-    coop_sums<false, C::GZ>(C::FUN(), C::FUN_COEF(), C::FUN_STEPS, lane);
+    coop_sums<false, C::GZ>(C::FUN(), C::FUN_COEF(), C::FUN_COEF2(), C::FUN_STEPS, lane);
     sync();
     vec([&](int i) { out[i] = lds_ld(C::GZ + C::FO + 8u * (unsigned)i); });
     sync();
@@ -317,7 +330,7 @@ struct CoopCell {
                                       (gcdouble*)row(L::RCT), (gcdouble*)row(L::FIX), cst(), T,
                                       lane);
     sync();
-    coop_sums<true, C::GZ>(C::JAC(), C::JAC_COEF(), C::JAC_STEPS, lane);
+    coop_sums<true, C::GZ>(C::JAC(), C::JAC_COEF(), C::JAC_COEF2(), C::JAC_STEPS, lane);
     sync();
     int ising = 0x7FFFFFFF;
     gcu32* dg = (gcu32*)C::DIAG();

@@ -101,26 +101,32 @@ static void products(const unsigned* P, int N, int NPAD, const double* V, const 
     at(C::GZ + 16 + 8 * q) = p;
   }
 }
-// a sum program in lockstep, as the device runs it: whole rounds of RING
-// steps, each step's product read two steps ahead (before the stores of the
-// two steps in between), every step storing
-static void sums(const unsigned long long* prog, const double* coef, int steps, bool jvs) {
+// a sum program in lockstep (two terms per step), as the device runs it:
+// whole rounds of RING steps, each step's products read two steps ahead
+// (before the stores of the two steps in between), every step storing
+static void sums(const unsigned long long* prog, const double* coef, const double* coef2,
+                 int steps, bool jvs) {
   const unsigned sb = jvs ? 0u : C::GZ;
   const int n = rounds(steps);
   if (n + 2 > steps + C::PAD) abort();
-  double acc[64] = {0}, x0[64], x1[64], x2[64];
+  double acc[64] = {0}, x0[64], x1[64], x2[64], y0[64], y1[64], y2[64];
   auto prod = [&](int t, int l) { return at(C::GZ + ((unsigned)prog[t * 64 + l] & 0xFFFFu)); };
-  for (int l = 0; l < 64; l++) { x0[l] = prod(0, l); x1[l] = prod(1, l); }
+  auto prod2 = [&](int t, int l) {
+    return at(C::GZ + ((unsigned)(prog[t * 64 + l] >> 32) & 0xFFFFu));
+  };
+  for (int l = 0; l < 64; l++) {
+    x0[l] = prod(0, l); x1[l] = prod(1, l); y0[l] = prod2(0, l); y1[l] = prod2(1, l);
+  }
   for (int t = 0; t < n; t++) {
-    for (int l = 0; l < 64; l++) x2[l] = prod(t + 2, l);
+    for (int l = 0; l < 64; l++) { x2[l] = prod(t + 2, l); y2[l] = prod2(t + 2, l); }
     for (int l = 0; l < 64; l++) {
       const unsigned long long d = prog[t * 64 + l];
       const double term = coef[t * 64 + l] * x0[l];
       const double sum = acc[l] + term;
-      acc[l] = ((int)(unsigned)(d >> 32) < 0) ? term : sum;
+      const double v1 = ((long long)d < 0) ? term : sum;
+      acc[l] = v1 + coef2[t * 64 + l] * y0[l];
       at(sb + ((unsigned)d >> 16)) = jvs ? -acc[l] : acc[l];
-      x0[l] = x1[l];
-      x1[l] = x2[l];
+      x0[l] = x1[l]; x1[l] = x2[l]; y0[l] = y1[l]; y1[l] = y2[l];
     }
   }
   zero_slot_intact();
@@ -128,13 +134,13 @@ static void sums(const unsigned long long* prog, const double* coef, int steps, 
 extern "C" void coop_fun(double* V, double* F, double* R, double* out) {
   fresh();
   products(C::APROD(), C::NA, C::NA_PAD, V, F, R);
-  sums(C::FUN(), C::FUN_COEF(), C::FUN_STEPS, false);
+  sums(C::FUN(), C::FUN_COEF(), C::FUN_COEF2(), C::FUN_STEPS, false);
   for (int i = 0; i < M::NVAR; i++) out[i] = at(C::GZ + C::FO + 8 * i);
 }
 extern "C" void coop_jac(double* V, double* F, double* R, double ghinv, double* G) {
   fresh();
   products(C::BPROD(), C::NB, C::NB_PAD, V, F, R);
-  sums(C::JAC(), C::JAC_COEF(), C::JAC_STEPS, true);
+  sums(C::JAC(), C::JAC_COEF(), C::JAC_COEF2(), C::JAC_STEPS, true);
   for (int i = 0; i < M::NVAR; i++) at(8 * C::DIAG()[i]) = at(8 * C::DIAG()[i]) + ghinv;
   for (int n = 0; n < M::LU_NONZERO; n++) G[n] = at(8 * C::GSLOT()[n]);
 }
