@@ -217,7 +217,8 @@ int kpp_check_params_f90(const int ICNTRL_U[20], const double RCNTRL_U[20], cons
  *             KPP_REAL RelTol[], KPP_REAL AbsTol[], KPP_REAL RCNTRL[],
  *             int ICNTRL[], KPP_REAL RSTATUS[], int ISTATUS[])   int/sdirk.c:170
  * (and its INTEGRATE, :91-166) for every cell of a device batch, one lane per
- * cell, over [b->tstart, b->tend] (or tstart0/tend0):
+ * cell at a time (a resident grid whose lanes take cells from a work queue,
+ * in b->order when given), over [b->tstart, b->tend] (or tstart0/tend0):
  *  - ICNTRL/RCNTRL as SDIRK takes them (0-based, no merge): ICNTRL[1] 0 =
  *    vector tolerances, ICNTRL[2] method 0/1 Sdirk2a, 2 Sdirk2b, 3 Sdirk3a,
  *    4 Sdirk4a, 5 Sdirk4b (any other: Sdirk2a), [3] max steps (0: 200000),
@@ -247,7 +248,8 @@ int kpp_sdirk_dev(const char* mech, const kpp_batch* b, const int ICNTRL[20],
  *                   int ICNTRL[], KPP_REAL RSTATUS[], int ISTATUS[],
  *                   int* IERR)                         int/runge_kutta.c:203
  * (and its INTEGRATE, :114-200) for every cell of a device batch, one lane
- * per cell, over [b->tstart, b->tend] (or tstart0/tend0), with the complex
+ * per cell at a time (a work queue as kpp_sdirk_dev's), over [b->tstart,
+ * b->tend] (or tstart0/tend0), with the complex
  * sparse LU of util/sutil.c:28-87 on the device:
  *  - ICNTRL/RCNTRL as RungeKutta takes them (0-based, no merge): ICNTRL[1]
  *    0 = vector tolerances, [2] method 0/1 Radau-2A, 2 Lobatto-3C, 3 Gauss,
